@@ -1,0 +1,601 @@
+// 3-D convolutions of the 3DMatch descriptor network on gfx950 MFMA.
+//
+// Each convolution (core/marvin.hpp:4635-4709, cuDNN cross-correlation,
+// stride 1, no padding) is an implicit GEMM over NDHWC activations:
+//   rows  m = output voxels (patch, z, y, x)
+//   cols  n = output channels
+//   k     = (tap, input channel), tap-major, so for a fixed tap the k-run of
+//           a row is the contiguous channel vector of one input voxel.
+// Weights are packed [n][tap][c], so both operands are k-contiguous and a lane
+// loads its MFMA fragment with one 16-byte load:
+//   fp32  v_mfma_f32_16x16x4_f32: a 16-byte load holds 4 consecutive k; the
+//         four MFMA k-steps of a chunk consume them in turn;
+//   bf16  v_mfma_f32_16x16x32_bf16: a 16-byte load is the lane's 8 k.
+// Within a 16-row block lane l supplies row l%16 and k-group l/16; any
+// assignment of actual k to (k-group, step) is correct as long as the A and
+// B fragments use the same one, which they do by construction.
+//
+// A wave owns a 64x64 output tile (4x4 MFMA blocks, 64 accumulator VGPRs);
+// four waves form a workgroup stacked along m and/or n.  The loop over
+// (tap, channel chunk) prefetches the next chunk's fragments while the MFMAs
+// of the current one run.  Bias and ReLU (core/marvin.hpp:4655-4662,
+// 5483-5493) are fused into the epilogue; for conv2 the 2x2x2/2 max pool
+// (core/marvin.hpp:5578-5588) is fused too, by ordering rows so that each
+// aligned group of 8 rows is one pool window (rows 4g..4g+3 of a 16-row block
+// sit in one lane; the partner group is 16 lanes away).
+#include <algorithm>
+#include <cstring>
+
+#include "common.hpp"
+#include "conv.hpp"
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint16_t bf16_t;
+
+template <typename T>
+struct Kc;
+template <>
+struct Kc<float> {
+    static constexpr int value = 16;  // k per 16-byte lane load x 4 k-groups
+};
+template <>
+struct Kc<bf16_t> {
+    static constexpr int value = 32;
+};
+
+__device__ inline uint4 ld16(const void *p) { return *reinterpret_cast<const uint4 *>(p); }
+
+__device__ inline float comp(const uint4 &v, int s) {
+    unsigned u = s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
+    return __uint_as_float(u);
+}
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+    __device__ static inline void chunk(floatx4 (&acc)[4][4], const uint4 (&a)[4],
+                                        const uint4 (&b)[4]) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(comp(a[i], s), comp(b[j], s),
+                                                                     acc[i][j], 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<bf16_t> {
+    __device__ static inline void chunk(floatx4 (&acc)[4][4], const uint4 (&a)[4],
+                                        const uint4 (&b)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                bf16x8 av, bv;
+                __builtin_memcpy(&av, &a[i], 16);
+                __builtin_memcpy(&bv, &b[j], 16);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i][j], 0, 0, 0);
+            }
+    }
+};
+
+__device__ inline bf16_t to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+
+__device__ inline float from_bf16(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+template <typename OT>
+__device__ inline void store(OT *p, float v);
+template <>
+__device__ inline void store<float>(float *p, float v) {
+    *p = v;
+}
+template <>
+__device__ inline void store<bf16_t>(bf16_t *p, float v) {
+    *p = to_bf16(v);
+}
+
+struct ConvArgs {
+    const void *in;
+    const void *w;
+    const float *bias;
+    void *out;
+    float *partial;
+    long M;  // rows
+    int N, C, D, OD, ksize, taps, splits;
+    int relu;
+};
+
+// Input voxel index of output row m: rows are (b, z, y, x) in order, or, with
+// POOL, (b, pz, py, px, window) with the 2x2x2 window innermost.
+template <bool POOL>
+__device__ inline long row_voxel(long m, int D, int OD) {
+    int ox, oy, oz;
+    long b;
+    if (POOL) {
+        const int w = (int)(m & 7);
+        long q = m >> 3;
+        const int PD = OD >> 1;
+        const int px = (int)(q % PD);
+        q /= PD;
+        const int py = (int)(q % PD);
+        q /= PD;
+        const int pz = (int)(q % PD);
+        b = q / PD;
+        ox = 2 * px + (w & 1);
+        oy = 2 * py + ((w >> 1) & 1);
+        oz = 2 * pz + (w >> 2);
+    } else {
+        ox = (int)(m % OD);
+        long q = m / OD;
+        oy = (int)(q % OD);
+        q /= OD;
+        oz = (int)(q % OD);
+        b = q / OD;
+    }
+    return ((b * D + oz) * D + oy) * (long)D + ox;
+}
+
+template <typename T, typename OT, int WAVES_N, bool POOL, bool SPLIT>
+__global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
+    constexpr int KC = Kc<T>::value;
+    constexpr int WAVES_M = 4 / WAVES_N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / WAVES_N, wn = wave % WAVES_N;
+    const long m0 = ((long)blockIdx.x * WAVES_M + wm) * 64;
+    const int n0 = (blockIdx.y * WAVES_N + wn) * 64;
+    if (m0 >= p.M) return;  // no barriers below
+    const int r16 = lane & 15, g = lane >> 4;
+    const int lanek = g * (KC / 4);
+    const T *__restrict__ in = static_cast<const T *>(p.in);
+    const T *__restrict__ w = static_cast<const T *>(p.w);
+    const long K = (long)p.taps * p.C;
+
+    const T *ap[4];
+    const T *bp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        long m = min(m0 + 16 * i + r16, p.M - 1);
+        ap[i] = in + row_voxel<POOL>(m, p.D, p.OD) * p.C + lanek;
+        bp[i] = w + (long)(n0 + 16 * i + r16) * K + lanek;
+    }
+
+    const int cpt = p.C / KC;
+    const int nq = p.taps * cpt;
+    int q0 = 0, q1 = nq;
+    if (SPLIT) {
+        q0 = (int)((long)nq * blockIdx.z / p.splits);
+        q1 = (int)((long)nq * (blockIdx.z + 1) / p.splits);
+    }
+    const int D = p.D, C = p.C, ks = p.ksize;
+    auto offsets = [&](int q, long &aoff, long &boff) {
+        const int tap = q / cpt;
+        const int c0 = (q - tap * cpt) * KC;
+        int kz = 0, ky = 0, kx = 0;
+        if (ks == 3) {
+            kz = tap / 9;
+            ky = (tap / 3) % 3;
+            kx = tap % 3;
+        }
+        aoff = (long)((kz * D + ky) * D + kx) * C + c0;
+        boff = (long)tap * C + c0;
+    };
+
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    if (q0 < q1) {
+        uint4 a[4], b[4];
+        long aoff, boff;
+        offsets(q0, aoff, boff);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = ld16(ap[i] + aoff);
+            b[i] = ld16(bp[i] + boff);
+        }
+        for (int q = q0; q < q1; ++q) {
+            uint4 an[4], bn[4];
+            const bool more = q + 1 < q1;
+            if (more) {
+                offsets(q + 1, aoff, boff);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    an[i] = ld16(ap[i] + aoff);
+                    bn[i] = ld16(bp[i] + boff);
+                }
+            }
+            Mfma<T>::chunk(acc, a, b);
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    a[i] = an[i];
+                    b[i] = bn[i];
+                }
+            }
+        }
+    }
+
+    // Epilogue.  Lane holds rows m0+16i+4g+r (r = 0..3) of column n0+16j+r16.
+    if (SPLIT) {
+        float *__restrict__ part = p.partial + (long)blockIdx.z * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int col = n0 + 16 * j + r16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) part[row * p.N + col] = acc[i][j][r];
+                }
+            }
+        return;
+    }
+    OT *__restrict__ out = static_cast<OT *>(p.out);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = n0 + 16 * j + r16;
+        const float bv = p.bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = acc[i][j][r] + bv;
+                if (p.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+            }
+            if (POOL) {
+                float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                const long row = m0 + 16 * i + 4 * g;
+                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, mx);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) store<OT>(out + row * p.N + col, v[r]);
+                }
+            }
+        }
+    }
+}
+
+// Sum of split-K partials + bias (+ReLU), converted to the output type.
+template <typename OT>
+__global__ void split_reduce_kernel(const float *__restrict__ part, int splits, long MN, int N,
+                                    const float *__restrict__ bias, int relu, OT *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < MN;
+         i += (long)gridDim.x * blockDim.x) {
+        float s = part[i];
+        for (int k = 1; k < splits; ++k) s += part[(long)k * MN + i];
+        s += bias[i % N];
+        if (relu) s = s > 0.0f ? s : 0.0f;
+        store<OT>(out + i, s);
+    }
+}
+
+// ---- first convolution (one input channel) -----------------------------------------
+
+struct Conv1Args {
+    const float *base;
+    const int *kp;
+    int dx, dy, dz;
+    int P;  // patch extent; output extent P - 2
+    const void *w;
+    const float *bias;
+    void *out;
+    long M;
+    int N;
+    int relu;
+};
+
+// Value of input voxel (lx, ly, lz) of patch b (local patch coordinates).
+template <bool GRID>
+__device__ inline float fetch1(const Conv1Args &a, long b, int lx, int ly, int lz) {
+    if (GRID) {
+        const int h = a.P / 2;
+        const int x = a.kp[b * 3 + 0] - h + lx, y = a.kp[b * 3 + 1] - h + ly,
+                  z = a.kp[b * 3 + 2] - h + lz;
+        if (x < 0 || y < 0 || z < 0 || x >= a.dx || y >= a.dy || z >= a.dz) return 0.0f;
+        return a.base[((long)z * a.dy + y) * a.dx + x];
+    }
+    return a.base[b * (long)a.P * a.P * a.P + ((long)lz * a.P + ly) * a.P + lx];
+}
+
+template <typename T, typename OT, bool GRID>
+__global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
+    const int n0 = blockIdx.y * 64;
+    if (m0 >= a.M) return;
+    const int r16 = lane & 15, g = lane >> 4;
+    const int OD = a.P - 2;
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int KP = F32 ? 28 : 32;
+
+    long pb[4];
+    int px[4], py[4], pz[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        long m = min(m0 + 16 * i + r16, a.M - 1);
+        px[i] = (int)(m % OD);
+        long q = m / OD;
+        py[i] = (int)(q % OD);
+        q /= OD;
+        pz[i] = (int)(q % OD);
+        pb[i] = q / OD;
+    }
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const T *__restrict__ w = static_cast<const T *>(a.w);
+    if (F32) {
+        // 7 k-steps of 4 taps: lane group g supplies tap 4s + g.
+#pragma unroll
+        for (int s = 0; s < 7; ++s) {
+            const int t = 4 * s + g;
+            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+            float av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                av[i] = t < 27 ? fetch1<GRID>(a, pb[i], px[i] + kx, py[i] + ky, pz[i] + kz) : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float *wf = reinterpret_cast<const float *>(w);
+                bv[j] = wf[(long)(n0 + 16 * j + r16) * KP + t];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    } else {
+        // One k-step of 32 taps: lane group g supplies taps 8g .. 8g+7.
+        uint4 af[4], bf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            unsigned short hv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int t = 8 * g + e;
+                const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+                float v = t < 27 ? fetch1<GRID>(a, pb[i], px[i] + kx, py[i] + ky, pz[i] + kz) : 0.0f;
+                hv[e] = to_bf16(v);
+            }
+            __builtin_memcpy(&af[i], hv, 16);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bf16_t *wb = reinterpret_cast<const bf16_t *>(w);
+            bf[j] = ld16(wb + (long)(n0 + 16 * j + r16) * KP + 8 * g);
+        }
+        Mfma<bf16_t>::chunk(acc, af, bf);
+    }
+
+    OT *__restrict__ out = static_cast<OT *>(a.out);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = n0 + 16 * j + r16;
+        const float bv = a.bias[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long row = m0 + 16 * i + 4 * g + r;
+                float v = acc[i][j][r] + bv;
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                if (row < a.M) store<OT>(out + row * a.N + col, v);
+            }
+    }
+}
+
+// ---- layout helpers -------------------------------------------------------------------
+
+template <typename T>
+__device__ inline float load_as_float(const T *p);
+template <>
+__device__ inline float load_as_float<float>(const float *p) {
+    return *p;
+}
+template <>
+__device__ inline float load_as_float<bf16_t>(const bf16_t *p) {
+    return from_bf16(*p);
+}
+
+template <typename T>
+__global__ void to_ncdhw_kernel(const T *__restrict__ in, long batch, int C, int E,
+                                float *__restrict__ out) {
+    const long vol = (long)E * E * E;
+    const long total = batch * C * vol;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const long v = i % vol;
+        const long bc = i / vol;
+        const int c = (int)(bc % C);
+        const long b = bc / C;
+        out[i] = load_as_float<T>(in + (b * vol + v) * C + c);
+    }
+}
+
+template <typename T>
+__global__ void maxpool_kernel(const T *__restrict__ in, long batch, int C, int E, int OE,
+                               T *__restrict__ out) {
+    const long total = batch * OE * OE * OE * (long)C;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long q = i / C;
+        const int x = (int)(q % OE);
+        q /= OE;
+        const int y = (int)(q % OE);
+        q /= OE;
+        const int z = (int)(q % OE);
+        const long b = q / OE;
+        float m = -INFINITY;
+        for (int kz = 0; kz < 2; ++kz)
+            for (int ky = 0; ky < 2; ++ky)
+                for (int kx = 0; kx < 2; ++kx) {
+                    const int iz = 2 * z + kz, iy = 2 * y + ky, ix = 2 * x + kx;
+                    if (iz >= E || iy >= E || ix >= E) continue;
+                    m = fmaxf(m, load_as_float<T>(in + (((b * E + iz) * E + iy) * E + ix) * C + c));
+                }
+        store<T>(out + i, m);
+    }
+}
+
+template <typename T, typename OT>
+int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void *w,
+                  const float *bias, void *out, int splits, float *partial, hipStream_t s) {
+    ConvArgs p;
+    p.in = in;
+    p.w = w;
+    p.bias = bias;
+    p.out = out;
+    p.partial = partial;
+    p.N = d.N;
+    p.C = d.C;
+    p.D = d.D;
+    p.OD = d.OD;
+    p.ksize = d.ksize;
+    p.taps = d.ksize * d.ksize * d.ksize;
+    p.splits = splits;
+    p.relu = d.relu ? 1 : 0;
+    p.M = (long)batch * d.OD * d.OD * d.OD;
+    const int waves_n = d.N % 256 == 0 ? 4 : d.N % 128 == 0 ? 2 : 1;
+    const int waves_m = 4 / waves_n;
+    dim3 grid((unsigned)((p.M + 64L * waves_m - 1) / (64L * waves_m)), d.N / (64 * waves_n),
+              splits);
+#define TDM_CONV_LAUNCH(WN)                                                                   \
+    do {                                                                                      \
+        if (splits > 1)                                                                       \
+            conv_kernel<T, OT, WN, false, true><<<grid, 256, 0, s>>>(p);                      \
+        else if (d.pool)                                                                      \
+            conv_kernel<T, OT, WN, true, false><<<grid, 256, 0, s>>>(p);                      \
+        else                                                                                  \
+            conv_kernel<T, OT, WN, false, false><<<grid, 256, 0, s>>>(p);                     \
+    } while (0)
+    if (waves_n == 4) TDM_CONV_LAUNCH(4);
+    else if (waves_n == 2) TDM_CONV_LAUNCH(2);
+    else TDM_CONV_LAUNCH(1);
+#undef TDM_CONV_LAUNCH
+    TDM_LAUNCHED();
+    if (splits > 1) {
+        const long MN = p.M * d.N;
+        unsigned blocks = (unsigned)std::min<long>((MN + 255) / 256, 16384);
+        split_reduce_kernel<OT><<<blocks, 256, 0, s>>>(partial, splits, MN, d.N, bias, p.relu,
+                                                       static_cast<OT *>(out));
+        TDM_LAUNCHED();
+    }
+    return TDM_OK;
+}
+
+}  // namespace
+
+namespace tdm {
+
+uint16_t float_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void *in,
+                const void *w, const float *bias, void *out, int splits, float *partial,
+                hipStream_t s) {
+    TDM_CHECK(d.C % 32 == 0 && d.N % 64 == 0, TDM_ERR_UNSUPPORTED,
+              "convolution with %d->%d channels (need multiples of 32 and 64)", d.C, d.N);
+    TDM_CHECK(d.ksize == 3 || d.ksize == 1, TDM_ERR_UNSUPPORTED, "window %d", d.ksize);
+    TDM_CHECK(d.OD == d.D - d.ksize + 1 && d.OD > 0, TDM_ERR_INVALID_ARG, "bad extents");
+    TDM_CHECK(!d.pool || (d.OD % 2 == 0 && splits <= 1), TDM_ERR_INVALID_ARG, "bad fused pool");
+    TDM_CHECK(splits >= 1 && splits <= 65535 && (splits == 1 || partial), TDM_ERR_INVALID_ARG,
+              "bad split");
+    if (batch <= 0) return TDM_OK;
+    if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, s);
+    if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, s);
+    return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, s);
+}
+
+int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool relu, int batch,
+                 const void *w, const float *bias, void *out, hipStream_t s) {
+    TDM_CHECK(N % 64 == 0, TDM_ERR_UNSUPPORTED, "first convolution needs a multiple of 64 outputs");
+    TDM_CHECK(src.P >= 3, TDM_ERR_INVALID_ARG, "patch extent %d", src.P);
+    if (batch <= 0) return TDM_OK;
+    Conv1Args a;
+    a.base = src.base;
+    a.kp = src.kp;
+    a.dx = src.dims[0];
+    a.dy = src.dims[1];
+    a.dz = src.dims[2];
+    a.P = src.P;
+    a.w = w;
+    a.bias = bias;
+    a.out = out;
+    a.N = N;
+    a.relu = relu ? 1 : 0;
+    const long OD = src.P - 2;
+    a.M = (long)batch * OD * OD * OD;
+    dim3 grid((unsigned)((a.M + 255) / 256), N / 64);
+    const bool gridmode = src.kp != nullptr;
+#define TDM_C1(T, OT)                                                                         \
+    do {                                                                                      \
+        if (gridmode) conv1_kernel<T, OT, true><<<grid, 256, 0, s>>>(a);                      \
+        else conv1_kernel<T, OT, false><<<grid, 256, 0, s>>>(a);                              \
+    } while (0)
+    if (dt == DType::F32) TDM_C1(float, float);
+    else if (out_f32) TDM_C1(bf16_t, float);
+    else TDM_C1(bf16_t, bf16_t);
+#undef TDM_C1
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int launch_to_ncdhw(DType dt, bool in_f32, const void *in, int batch, int C, int E, float *out,
+                    hipStream_t s) {
+    const long total = (long)batch * C * E * E * E;
+    if (total <= 0) return TDM_OK;
+    unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    if (dt == DType::F32 || in_f32)
+        to_ncdhw_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E, out);
+    else
+        to_ncdhw_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C, E, out);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int launch_maxpool(DType dt, const void *in, int batch, int C, int E, void *out, hipStream_t s) {
+    const int OE = 1 + (E - 2 + 1) / 2;  // 1 + ceil((E - 2) / 2)
+    const long total = (long)batch * OE * OE * OE * C;
+    if (total <= 0) return TDM_OK;
+    unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    if (dt == DType::F32)
+        maxpool_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E,
+                                                     OE, static_cast<float *>(out));
+    else
+        maxpool_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C,
+                                                      E, OE, static_cast<bf16_t *>(out));
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+}  // namespace tdm

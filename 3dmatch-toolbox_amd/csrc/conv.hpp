@@ -1,0 +1,64 @@
+// Internal interface between the network driver (net.cpp) and the HIP
+// convolution kernels (conv.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace tdm {
+
+enum class DType { F32 = 0, BF16 = 1 };
+
+inline size_t dtype_size(DType t) { return t == DType::F32 ? 4 : 2; }
+
+// One 3-D convolution layer in implicit-GEMM form over NDHWC activations:
+// rows = output voxels of `batch` cubes, cols = output channels, reduction =
+// taps x input channels (tap-major, channel-minor; weights packed [N][taps][C]).
+struct ConvDesc {
+    int C = 0;      // input channels (multiple of 32)
+    int N = 0;      // output channels (multiple of 64)
+    int D = 0;      // input extent
+    int OD = 0;     // output extent (before pooling)
+    int ksize = 3;  // window (3 or 1)
+    bool relu = true;
+    bool pool = false;  // fused 2x2x2 stride-2 max pool (OD even)
+};
+
+// Launches the convolution for `batch` cubes.  out is NDHWC with the output
+// type: float when out_f32, else the storage type `dt`.  `splits` > 1 splits
+// the reduction and needs `partial` (splits*rows*N floats); no pooling then.
+int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void *in,
+                const void *w, const float *bias, void *out, int splits, float *partial,
+                hipStream_t s);
+
+// Where the first (single-channel) convolution reads its input.
+struct VolumeSource {
+    const float *base = nullptr;  // patches [batch][P^3] or a TDF grid
+    const int *kp = nullptr;      // grid mode: keypoint grid coords [batch][3]
+    int dims[3] = {0, 0, 0};      // grid mode: grid dims
+    int P = 30;                   // patch extent
+};
+
+// First convolution: 1 input channel, N output channels (multiple of 64),
+// window 3, fused bias + ReLU; output NDHWC of type dt (float if out_f32).
+// w is packed [N][KP] with KP = 28 (F32) or 32 (BF16) taps, zero padded.
+int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool relu, int batch,
+                 const void *w, const float *bias, void *out, hipStream_t s);
+
+// Taps per output channel in the packed weights of the first convolution.
+inline int conv1_taps_padded(DType t) { return t == DType::F32 ? 28 : 32; }
+
+// NDHWC (type dt or float) -> NCDHW float, for per-layer outputs.
+int launch_to_ncdhw(DType dt, bool in_f32, const void *in, int batch, int C, int E, float *out,
+                    hipStream_t s);
+
+// Max pool window 2 stride 2 over NDHWC when it cannot be fused:
+// out extent 1 + ceil((E - 2)/2), windows clipped.
+int launch_maxpool(DType dt, const void *in, int batch, int C, int E, void *out, hipStream_t s);
+
+// Host float -> bf16 bits, round to nearest even.
+uint16_t float_to_bf16(float f);
+
+}  // namespace tdm

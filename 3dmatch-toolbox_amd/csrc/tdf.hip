@@ -1,0 +1,514 @@
+// Point cloud -> TDF on gfx950, and the 30^3 patch gather.
+//
+// Reference: core/demo.cu:22-57 (ComputeTDF) searches, for every voxel, the
+// whole 11^3 cube of a float occupancy grid (up to 1331 scattered loads per
+// empty voxel).  Here:
+//   1. points are scattered into a 1-bit occupancy grid (rows of 32-bit
+//      words, x fastest; 1/32 of the float grid's bytes);
+//   2. one workgroup per 64 x 16 x 16 output tile loads the bits of the tile
+//      and its r-voxel halo once and computes the truncated squared distance
+//      transform separably in LDS: nearest set bit along x with bit scans,
+//      then min over dy of (f + dy^2), then over dz;
+//   3. d^2 indexes a table of 1 - sqrtf(d^2)/r built on the host with the
+//      reference's float operations, so values are bit-exact without any
+//      device sqrt or division; tiles whose halo holds no point write zeros.
+// The minimum of dx^2+dy^2+dz^2 over occupied voxels of the clipped cube is
+// exactly what the reference's max over 1 - sqrt(d^2)/r selects, because the
+// table is non-increasing in d^2.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kMaxR = 8;          // largest supported search radius (voxels)
+constexpr int kLutSize = 3 * kMaxR * kMaxR + 1;
+constexpr int kTX = 64, kTY = 16, kTZ = 16;
+constexpr int kTdfThreads = 256;
+constexpr unsigned char kNone = 255;  // no occupied voxel within reach
+
+struct Lut {
+    float v[kLutSize];
+};
+
+// Host-side table, core/demo.cu:51-54: value of an occupied voxel at squared
+// distance d2, or 0 when 1 - dist is not positive (the grid starts at 0 and
+// only grows).
+Lut make_lut(int r) {
+    Lut l;
+    for (int d2 = 0; d2 < kLutSize; ++d2) {
+        volatile float fd2 = (float)d2;  // keep host float semantics explicit
+        float dist = sqrtf(fd2) / (float)r;
+        float v = 1.0f - dist;
+        l.v[d2] = v > 0.0f ? v : 0.0f;
+    }
+    return l;
+}
+
+__host__ __device__ inline long row_words(int dx) { return (dx + 31) / 32; }
+
+// ---- bounding box -------------------------------------------------------------
+
+__global__ void bbox_partial_kernel(const float *__restrict__ pts, int n, float *__restrict__ part) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float v = pts[(long)i * 3 + a];
+            lo[a] = fminf(lo[a], v);
+            hi[a] = fmaxf(hi[a], v);
+        }
+    }
+    __shared__ float s[6][256];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = lo[a];
+        s[3 + a][threadIdx.x] = hi[a];
+    }
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                s[a][threadIdx.x] = fminf(s[a][threadIdx.x], s[a][threadIdx.x + off]);
+                s[3 + a][threadIdx.x] = fmaxf(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void bbox_final_kernel(const float *__restrict__ part, int nblocks, float *__restrict__ out) {
+    int a = threadIdx.x;
+    if (a >= 6) return;
+    float v = part[a];
+    for (int b = 1; b < nblocks; ++b) v = a < 3 ? fminf(v, part[b * 6 + a]) : fmaxf(v, part[b * 6 + a]);
+    out[a] = v;
+}
+
+// ---- occupancy ---------------------------------------------------------------------
+
+__global__ void fill_u32_kernel(unsigned *__restrict__ p, size_t n, unsigned v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// core/demo.cu:129-134: voxel = round((p - origin)/vs) per axis.
+__device__ inline bool point_voxel(const float *__restrict__ pts, long i, int dx, int dy, int dz,
+                                   float ox, float oy, float oz, float vs, int &gx, int &gy,
+                                   int &gz) {
+    float fx = roundf((pts[i * 3 + 0] - ox) / vs);
+    float fy = roundf((pts[i * 3 + 1] - oy) / vs);
+    float fz = roundf((pts[i * 3 + 2] - oz) / vs);
+    if (!(fx >= 0.0f && fx < (float)dx && fy >= 0.0f && fy < (float)dy && fz >= 0.0f &&
+          fz < (float)dz))
+        return false;
+    gx = (int)fx;
+    gy = (int)fy;
+    gz = (int)fz;
+    return true;
+}
+
+__global__ void scatter_bits_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
+                                    float ox, float oy, float oz, float vs,
+                                    unsigned *__restrict__ bits) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int gx, gy, gz;
+    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
+    long w = ((long)gz * dy + gy) * row_words(dx) + (gx >> 5);
+    atomicOr(&bits[w], 1u << (gx & 31));
+}
+
+__global__ void scatter_occ_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
+                                   float ox, float oy, float oz, float vs, float *__restrict__ occ) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int gx, gy, gz;
+    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
+    occ[((long)gz * dy + gy) * dx + gx] = 1.0f;
+}
+
+// Float occupancy -> bit rows.  One wave handles 64 consecutive x of a row
+// (two words) with a ballot.
+__global__ void pack_bits_kernel(const float *__restrict__ occ, int dx, long rows,
+                                 unsigned *__restrict__ bits) {
+    const long rw = row_words(dx);
+    const long pairs_per_row = (rw + 1) / 2;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= rows * pairs_per_row) return;
+    const long row = wave / pairs_per_row;
+    const int x = (int)(wave % pairs_per_row) * 64 + lane;
+    bool on = x < dx && occ[row * dx + x] > 0.0f;
+    unsigned long long m = __ballot(on);
+    long w = row * rw + (x >> 5);
+    if (lane == 0) bits[w] = (unsigned)m;
+    if (lane == 32 && (x >> 5) < rw) bits[w] = (unsigned)(m >> 32);
+}
+
+// ---- TDF tile kernel --------------------------------------------------------------
+
+// Truncated squared-distance transform of one 64x16x16 tile.
+template <int R>
+__global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *__restrict__ bits,
+                                                               int dx, int dy, int dz, Lut lut,
+                                                               float *__restrict__ tdf) {
+    constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
+    __shared__ unsigned char s1[HZ * HY * kTX];  // pass x, all halo rows
+    __shared__ unsigned char s2[HZ * kTY * kTX]; // pass y
+    __shared__ float slut[kLutSize];
+    __shared__ int any_bits;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int X0 = blockIdx.x * kTX, Y0 = blockIdx.y * kTY, Z0 = blockIdx.z * kTZ;
+    const long rw = row_words(dx);
+    const long wbase = X0 >> 5;  // first word of the tile in a row
+
+    if (threadIdx.x == 0) any_bits = 0;
+    for (int i = threadIdx.x; i < kLutSize; i += kTdfThreads) slut[i] = lut.v[i];
+    __syncthreads();
+
+    // Pass x over every halo row: nearest set bit within R along x.
+    bool seen = false;
+    for (int row = wave; row < HY * HZ; row += kTdfThreads / 64) {
+        const int yy = row % HY, zz = row / HY;
+        const int y = Y0 + yy - R, z = Z0 + zz - R;
+        unsigned char f = kNone;
+        if (y >= 0 && y < dy && z >= 0 && z < dz) {
+            const unsigned *r = bits + ((long)z * dy + y) * rw;
+            // Words covering x in [X0-32, X0+96).
+            unsigned w0 = wbase - 1 >= 0 ? r[wbase - 1] : 0u;
+            unsigned w1 = wbase < rw ? r[wbase] : 0u;
+            unsigned w2 = wbase + 1 < rw ? r[wbase + 1] : 0u;
+            unsigned w3 = wbase + 2 < rw ? r[wbase + 2] : 0u;
+            if (w0 | w1 | w2 | w3) {
+                seen = true;
+                unsigned long long lo = ((unsigned long long)w1 << 32) | w0;
+                unsigned long long hi = ((unsigned long long)w3 << 32) | w2;
+                // Window of 2R+1 bits starting at local bit lane + 32 - R.
+                const int s = lane + 32 - R;
+                unsigned long long win;
+                if (s == 0) win = lo;
+                else if (s < 64) win = (lo >> s) | (hi << (64 - s));
+                else win = hi >> (s - 64);
+                win &= (1ull << (2 * R + 1)) - 1;
+                if (win) {
+                    unsigned long long right = win >> R;  // bits at >= x
+                    unsigned long long left = win & ((2ull << R) - 1);  // bits at <= x
+                    int d = 255;
+                    if (right) d = __builtin_ctzll(right);
+                    if (left) d = min(d, R - (63 - __builtin_clzll(left)));
+                    f = (unsigned char)(d * d);
+                }
+            }
+        }
+        s1[row * kTX + lane] = f;
+    }
+    if (seen) any_bits = 1;  // benign race: every writer stores 1
+    __syncthreads();
+
+    const bool empty = any_bits == 0;
+    if (!empty) {
+        // Pass y: rows (zz in halo, y in tile).
+        for (int row = wave; row < HZ * kTY; row += kTdfThreads / 64) {
+            const int y = row % kTY, zz = row / kTY;
+            int best = kNone;
+#pragma unroll
+            for (int d = -R; d <= R; ++d) {
+                int v = s1[(zz * HY + y + R + d) * kTX + lane];
+                if (v != kNone) best = min(best, v + d * d);
+            }
+            s2[row * kTX + lane] = (unsigned char)min(best, (int)kNone);
+        }
+        __syncthreads();
+    }
+
+    // Pass z and the table lookup; write the tile.
+    const int x = X0 + lane;
+    for (int row = wave; row < kTY * kTZ; row += kTdfThreads / 64) {
+        const int y = row % kTY, zl = row / kTY;
+        const int gy = Y0 + y, gz = Z0 + zl;
+        if (gy >= dy || gz >= dz || x >= dx) continue;
+        float val = 0.0f;
+        if (!empty) {
+            int best = kNone;
+#pragma unroll
+            for (int d = -R; d <= R; ++d) {
+                int v = s2[((zl + R + d) * kTY + y) * kTX + lane];
+                if (v != kNone) best = min(best, v + d * d);
+            }
+            if (best < kLutSize) val = slut[best];
+        }
+        tdf[((long)gz * dy + gy) * dx + x] = val;
+    }
+}
+
+// float32 decode of core/demo.cu:32-34.
+__device__ inline void float_decode(long v, int dx, int dy, int &x, int &y, int &z) {
+    int vi = (int)v;
+    z = (int)floorf((float)vi / ((float)dx * (float)dy));
+    y = (int)floorf(((float)vi - ((float)z * (float)dx * (float)dy)) / (float)dx);
+    x = (int)((float)vi - ((float)z * (float)dx * (float)dy) - ((float)y * (float)dx));
+}
+
+__device__ inline bool bit_at(const unsigned *__restrict__ bits, long rw, int dy, int x, int y,
+                              int z) {
+    return (bits[((long)z * dy + y) * rw + (x >> 5)] >> (x & 31)) & 1u;
+}
+
+// Reference decode: voxels whose float decode differs from the true
+// coordinates search around the decoded coordinates instead (core/demo.cu:
+// 44-56 with the clipped bounds), occupancy still taken at the true index.
+__global__ void tdf_refdecode_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
+                                     int r, Lut lut, float *__restrict__ tdf) {
+    const long n = (long)dx * dy * dz;
+    const long rw = row_words(dx);
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long)gridDim.x * blockDim.x) {
+        int x, y, z;
+        float_decode(v, dx, dy, x, y, z);
+        const int tx = (int)(v % dx), ty = (int)((v / dx) % dy), tz = (int)(v / ((long)dx * dy));
+        if (x == tx && y == ty && z == tz) continue;
+        if (bit_at(bits, rw, dy, tx, ty, tz)) continue;  // occupied: 1, already set
+        int best = kLutSize;
+        for (int ix = max(0, x - r); ix < min(dx, x + r + 1); ++ix)
+            for (int iy = max(0, y - r); iy < min(dy, y + r + 1); ++iy)
+                for (int iz = max(0, z - r); iz < min(dz, z + r + 1); ++iz)
+                    if (bit_at(bits, rw, dy, ix, iy, iz)) {
+                        int d2 = (x - ix) * (x - ix) + (y - iy) * (y - iy) + (z - iz) * (z - iz);
+                        best = min(best, d2);
+                    }
+        tdf[v] = best < kLutSize ? lut.v[best] : 0.0f;
+    }
+}
+
+// ---- patch gather --------------------------------------------------------------------
+
+__global__ void extract_patches_kernel(const float *__restrict__ tdf, int dx, int dy, int dz,
+                                       const int *__restrict__ kp, int num, int size,
+                                       float *__restrict__ out) {
+    const long vol = (long)size * size * size;
+    const long total = vol * num;
+    const int h = size / 2;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const long p = i / vol;
+        const int li = (int)(i % vol);
+        const int lx = li % size, ly = (li / size) % size, lz = li / (size * size);
+        const int x = kp[p * 3 + 0] - h + lx, y = kp[p * 3 + 1] - h + ly, z = kp[p * 3 + 2] - h + lz;
+        float v = 0.0f;
+        if (x >= 0 && y >= 0 && z >= 0 && x < dx && y < dy && z < dz)
+            v = tdf[((long)z * dy + y) * dx + x];
+        out[i] = v;
+    }
+}
+
+// ---- drivers -----------------------------------------------------------------------
+
+int check_dims(int dx, int dy, int dz) {
+    TDM_CHECK(dx > 0 && dy > 0 && dz > 0, TDM_ERR_INVALID_ARG, "bad grid %dx%dx%d", dx, dy, dz);
+    TDM_CHECK((double)dx * dy * dz <= 2147483647.0, TDM_ERR_INVALID_ARG,
+              "grid of %dx%dx%d exceeds 2^31 voxels", dx, dy, dz);
+    return TDM_OK;
+}
+
+int search_radius(float vs, float trunc, int *r) {
+    TDM_CHECK(vs > 0 && trunc > 0, TDM_ERR_INVALID_ARG, "voxel size and truncation must be > 0");
+    volatile float q = trunc / vs;  // core/demo.cu:36, float division then round
+    int rr = (int)std::round(q);
+    TDM_CHECK(rr >= 1 && rr <= kMaxR, TDM_ERR_UNSUPPORTED,
+              "search radius %d outside [1, %d] voxels", rr, kMaxR);
+    *r = rr;
+    return TDM_OK;
+}
+
+template <int R>
+void launch_tile(const unsigned *bits, int dx, int dy, int dz, const Lut &lut, float *tdf,
+                 hipStream_t s) {
+    dim3 grid((dx + kTX - 1) / kTX, (dy + kTY - 1) / kTY, (dz + kTZ - 1) / kTZ);
+    tdf_tile_kernel<R><<<grid, kTdfThreads, 0, s>>>(bits, dx, dy, dz, lut, tdf);
+}
+
+// TDF from a bit grid already in scratch.
+int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decode, float *tdf,
+                  hipStream_t s) {
+    Lut lut = make_lut(r);
+    switch (r) {
+    case 1: launch_tile<1>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 2: launch_tile<2>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 3: launch_tile<3>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 4: launch_tile<4>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 5: launch_tile<5>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 6: launch_tile<6>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 7: launch_tile<7>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 8: launch_tile<8>(bits, dx, dy, dz, lut, tdf, s); break;
+    default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", r);
+    }
+    TDM_LAUNCHED();
+    if (decode == TDM_DECODE_REFERENCE) {
+        long n = (long)dx * dy * dz;
+        unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
+        tdf_refdecode_kernel<<<blocks, 256, 0, s>>>(bits, dx, dy, dz, r, lut, tdf);
+        TDM_LAUNCHED();
+    }
+    return TDM_OK;
+}
+
+int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, unsigned **bits, size_t *words,
+                 hipStream_t s) {
+    *words = (size_t)row_words(dx) * dy * dz;
+    void *p = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, *words * sizeof(unsigned), &p, s));
+    *bits = static_cast<unsigned *>(p);
+    unsigned blocks = (unsigned)std::min<size_t>(tdm::blocks_for(*words, 256), 16384);
+    fill_u32_kernel<<<blocks, 256, 0, s>>>(*bits, *words, 0u);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_size, int padding,
+                    int dims_out[3], float origin_out[3], void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(pts && dims_out && origin_out, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(num_pts > 0, TDM_ERR_INVALID_ARG, "empty point cloud");
+    TDM_CHECK(voxel_size > 0 && padding >= 0, TDM_ERR_INVALID_ARG, "bad voxel size / padding");
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    const int nblocks = (int)std::min<long>(((long)num_pts + 255) / 256, 1024);
+    void *scratch = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 1, sizeof(float) * 6 * (nblocks + 1), &scratch, s));
+    float *part = static_cast<float *>(scratch);
+    float *res = part + 6 * nblocks;
+    bbox_partial_kernel<<<nblocks, 256, 0, s>>>(pts, num_pts, part);
+    TDM_LAUNCHED();
+    bbox_final_kernel<<<1, 64, 0, s>>>(part, nblocks, res);
+    TDM_LAUNCHED();
+    float box[6];
+    TDM_HIP(hipMemcpyAsync(box, res, sizeof box, hipMemcpyDeviceToHost, s));
+    TDM_HIP(hipStreamSynchronize(s));
+    // core/demo.cu:116-122, float arithmetic on the host.
+    for (int a = 0; a < 3; ++a) {
+        TDM_CHECK(std::isfinite(box[a]) && std::isfinite(box[3 + a]), TDM_ERR_INVALID_ARG,
+                  "non-finite point coordinates");
+        volatile float ext = (box[3 + a] - box[a]) / voxel_size;
+        double dim = std::round((float)ext) + 1.0 + padding * 2.0;
+        TDM_CHECK(dim < 2147483647.0, TDM_ERR_INVALID_ARG, "grid extent too large");
+        dims_out[a] = (int)dim;
+        volatile float padw = padding * voxel_size;
+        volatile float o = box[a] - padw;
+        origin_out[a] = o + voxel_size / 2;
+    }
+    return TDM_OK;
+}
+
+int tdm_occupancy(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                  const float origin[3], float voxel_size, float *occ, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(pts && dims && origin && occ, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(num_pts >= 0 && voxel_size > 0, TDM_ERR_INVALID_ARG, "bad arguments");
+    TDM_TRY(check_dims(dims[0], dims[1], dims[2]));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    unsigned blocks = (unsigned)std::min<size_t>(tdm::blocks_for(n, 256), 16384);
+    fill_u32_kernel<<<blocks, 256, 0, s>>>(reinterpret_cast<unsigned *>(occ), n, 0u);
+    TDM_LAUNCHED();
+    if (num_pts == 0) return TDM_OK;
+    scatter_occ_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+        pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size, occ);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_compute_tdf(tdm_ctx *ctx, const float *occ, float *tdf, int dim_x, int dim_y, int dim_z,
+                    float voxel_size, float trunc_margin, int decode, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(occ && tdf, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
+              "bad decode mode %d", decode);
+    TDM_TRY(check_dims(dim_x, dim_y, dim_z));
+    int r = 0;
+    TDM_TRY(search_radius(voxel_size, trunc_margin, &r));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    unsigned *bits = nullptr;
+    size_t words = 0;
+    TDM_TRY(bits_scratch(ctx, dim_x, dim_y, dim_z, &bits, &words, s));
+    const long rows = (long)dim_y * dim_z;
+    const long pairs = (row_words(dim_x) + 1) / 2;
+    const long waves = rows * pairs;
+    pack_bits_kernel<<<tdm::blocks_for(waves * 64, 256), 256, 0, s>>>(occ, dim_x, rows, bits);
+    TDM_LAUNCHED();
+    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, s);
+}
+
+int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                        const float origin[3], float voxel_size, float trunc_margin, int decode,
+                        float *tdf, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(pts && dims && origin && tdf, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(num_pts >= 0, TDM_ERR_INVALID_ARG, "negative point count");
+    TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
+              "bad decode mode %d", decode);
+    TDM_TRY(check_dims(dims[0], dims[1], dims[2]));
+    int r = 0;
+    TDM_TRY(search_radius(voxel_size, trunc_margin, &r));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    unsigned *bits = nullptr;
+    size_t words = 0;
+    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], &bits, &words, s));
+    if (num_pts > 0) {
+        scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            bits);
+        TDM_LAUNCHED();
+    }
+    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, s);
+}
+
+int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
+                          int *grid_out) {
+    tdm::clear_error();
+    TDM_CHECK(num >= 0, TDM_ERR_INVALID_ARG, "negative count");
+    TDM_CHECK(num == 0 || (kp && origin && grid_out), TDM_ERR_INVALID_ARG, "null pointer");
+    TDM_CHECK(voxel_size > 0, TDM_ERR_INVALID_ARG, "bad voxel size");
+    for (long i = 0; i < num; ++i)
+        for (int a = 0; a < 3; ++a) {
+            volatile float q = (kp[i * 3 + a] - origin[a]) / voxel_size;
+            float r = std::round((float)q);
+            TDM_CHECK(std::fabs(r) < 2.0e9f, TDM_ERR_INVALID_ARG, "keypoint %ld out of range", i);
+            grid_out[i * 3 + a] = (int)r;
+        }
+    return TDM_OK;
+}
+
+int tdm_extract_patches(tdm_ctx *ctx, const float *tdf, int dim_x, int dim_y, int dim_z,
+                        const int *kp_grid, int num, int patch_size, float *patches, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(num >= 0 && patch_size > 0 && patch_size <= 256, TDM_ERR_INVALID_ARG, "bad sizes");
+    TDM_CHECK(num == 0 || (tdf && kp_grid && patches), TDM_ERR_INVALID_ARG, "null pointer");
+    TDM_TRY(check_dims(dim_x, dim_y, dim_z));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    if (num == 0) return TDM_OK;
+    long total = (long)patch_size * patch_size * patch_size * num;
+    unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    extract_patches_kernel<<<blocks, 256, 0, tdm::as_stream(stream)>>>(
+        tdf, dim_x, dim_y, dim_z, kp_grid, num, patch_size, patches);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// TSDF depth fusion on gfx950.
+//
+// The reference kernel (depth-fusion/demo.cu:18-68) runs one thread per
+// (z, y) column with a serial loop over x, so the 64 lanes of a wave touch
+// addresses dim_x*4 bytes apart.  Here a lane owns one (x, y) and sweeps a
+// run of z: the lanes of a wave cover consecutive x, every grid access is a
+// coalesced 256-byte row segment, and the per-column part of the camera
+// transform is computed once per lane instead of once per voxel.
+//
+// Bit-exactness: this file is compiled with -ffp-contract=off and IEEE
+// division/sqrt, and every per-voxel expression keeps the reference's
+// operation order, so pixel indices, the skip tests and the weights are
+// bit-identical and the TSDF values equal the reference arithmetic.
+// powf(v, 2) of the reference is evaluated as v*v (the single correctly
+// rounded product that gcc and clang both fold it to).
+#include "common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kZRun = 16;  // z slices swept by one lane
+
+// Per-frame camera constants as the kernel uses them.
+struct Cam {
+    float r00, r10, r20, tx;  // cam2base[0], [4], [8], [3]
+    float r01, r11, r21, ty;  // cam2base[1], [5], [9], [7]
+    float r02, r12, r22, tz;  // cam2base[2], [6], [10], [11]
+    float fx, cx, fy, cy;     // K[0], K[2], K[4], K[5]
+};
+
+__device__ inline Cam load_cam(const float *__restrict__ K, const float *__restrict__ m) {
+    Cam c;
+    c.r00 = m[0]; c.r10 = m[4]; c.r20 = m[8];  c.tx = m[3];
+    c.r01 = m[1]; c.r11 = m[5]; c.r21 = m[9];  c.ty = m[7];
+    c.r02 = m[2]; c.r12 = m[6]; c.r22 = m[10]; c.tz = m[11];
+    c.fx = K[0]; c.cx = K[2]; c.fy = K[4]; c.cy = K[5];
+    return c;
+}
+
+// Column constants of one (x, y) for one frame: the first two terms of each
+// row of R^T (p - t), summed left to right as the reference does.
+struct ColPart {
+    float sx, sy, sz;
+};
+
+__device__ inline ColPart col_part(const Cam &c, float bx, float by) {
+    float t0 = bx - c.tx;
+    float t1 = by - c.ty;
+    ColPart p;
+    p.sx = c.r00 * t0 + c.r10 * t1;
+    p.sy = c.r01 * t0 + c.r11 * t1;
+    p.sz = c.r02 * t0 + c.r12 * t1;
+    return p;
+}
+
+// The per-voxel body of depth-fusion/demo.cu:34-58 for one frame.
+// Returns true and the truncated signed distance when the voxel is updated.
+__device__ inline bool voxel_sdf(const Cam &c, const ColPart &p, float bz,
+                                 const float *__restrict__ depth, int H, int W, float trunc,
+                                 float &dist) {
+    float t2 = bz - c.tz;
+    float pcx = p.sx + c.r20 * t2;
+    float pcy = p.sy + c.r21 * t2;
+    float pcz = p.sz + c.r22 * t2;
+    if (!(pcz > 0.0f)) return false;
+    float qx = pcx / pcz;
+    float qy = pcy / pcz;
+    // roundf = half away from zero; the range test runs on the rounded float,
+    // which rejects exactly what the reference's int conversion + test rejects.
+    float rx = roundf(c.fx * qx + c.cx);
+    float ry = roundf(c.fy * qy + c.cy);
+    if (!(rx >= 0.0f && rx < (float)W && ry >= 0.0f && ry < (float)H)) return false;
+    int px = (int)rx, py = (int)ry;
+    float d = depth[py * W + px];
+    if (d <= 0.0f || d > 6.0f) return false;
+    float diff = (d - pcz) * sqrtf(1.0f + qx * qx + qy * qy);
+    if (diff <= -trunc) return false;
+    dist = fminf(1.0f, diff / trunc);
+    return true;
+}
+
+// One frame; lane = (x, y), blockIdx.y = run of kZRun z slices.
+__global__ __launch_bounds__(kThreads) void integrate_frame_kernel(
+    const float *__restrict__ K, const float *__restrict__ cam2base,
+    const float *__restrict__ depth, int H, int W, int dx, int dy, int dz, float ox, float oy,
+    float oz, float vs, float trunc, float *__restrict__ tsdf, float *__restrict__ weight) {
+    const long plane = (long)dx * dy;
+    const long col = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (col >= plane) return;
+    const int x = (int)(col % dx);
+    const int y = (int)(col / dx);
+    const Cam c = load_cam(K, cam2base);
+    const ColPart p = col_part(c, ox + x * vs, oy + y * vs);
+    const int z0 = blockIdx.y * kZRun;
+    const int z1 = min(dz, z0 + kZRun);
+    for (int z = z0; z < z1; ++z) {
+        float dist;
+        if (!voxel_sdf(c, p, oz + z * vs, depth, H, W, trunc, dist)) continue;
+        const long i = z * plane + col;
+        float w_old = weight[i];
+        float w_new = w_old + 1.0f;
+        weight[i] = w_new;
+        tsdf[i] = (tsdf[i] * w_old + dist) / w_new;
+    }
+}
+
+// All frames in one pass: a lane keeps the kZRun voxels of its run in
+// registers, applies the frames in order, reads a voxel the first time it is
+// updated and writes back only updated voxels.  Per voxel the sequence of
+// updates is the reference's, so the result is bit-identical.
+__global__ __launch_bounds__(kThreads) void integrate_frames_kernel(
+    const float *__restrict__ K, const float *__restrict__ cam2base,
+    const float *__restrict__ depths, int F, int H, int W, int dx, int dy, int dz, float ox,
+    float oy, float oz, float vs, float trunc, float *__restrict__ tsdf,
+    float *__restrict__ weight) {
+    const long plane = (long)dx * dy;
+    const long col = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (col >= plane) return;
+    const int x = (int)(col % dx);
+    const int y = (int)(col / dx);
+    const float bx = ox + x * vs;
+    const float by = oy + y * vs;
+    const int z0 = blockIdx.y * kZRun;
+    const int nz = min(dz - z0, kZRun);
+    const long base = z0 * plane + col;
+    float t[kZRun], w[kZRun];
+    unsigned loaded = 0;
+    const long frame_px = (long)H * W;
+    for (int f = 0; f < F; ++f) {
+        const Cam c = load_cam(K, cam2base + f * 16);
+        const ColPart p = col_part(c, bx, by);
+        const float *__restrict__ depth = depths + f * frame_px;
+#pragma unroll
+        for (int j = 0; j < kZRun; ++j) {
+            float dist;
+            if (j >= nz || !voxel_sdf(c, p, oz + (z0 + j) * vs, depth, H, W, trunc, dist)) continue;
+            if (!(loaded & (1u << j))) {
+                t[j] = tsdf[base + j * plane];
+                w[j] = weight[base + j * plane];
+                loaded |= 1u << j;
+            }
+            float w_new = w[j] + 1.0f;
+            t[j] = (t[j] * w[j] + dist) / w_new;
+            w[j] = w_new;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j) {
+        if (loaded & (1u << j)) {
+            tsdf[base + j * plane] = t[j];
+            weight[base + j * plane] = w[j];
+        }
+    }
+}
+
+__global__ void tsdf_init_kernel(float *__restrict__ tsdf, float *__restrict__ weight, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        tsdf[i] = 1.0f;
+        weight[i] = 0.0f;
+    }
+}
+
+// depth-fusion/utils.hpp:80-84: (float)mm / 1000.0f, above 6 m -> 0.
+__global__ void depth_from_mm_kernel(const uint16_t *__restrict__ mm, float *__restrict__ out,
+                                     size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d = (float)mm[i] / 1000.0f;
+    out[i] = d > 6.0f ? 0.0f : d;
+}
+
+int check_grid(int H, int W, int dx, int dy, int dz, float vs, float trunc) {
+    TDM_CHECK(H > 0 && W > 0, TDM_ERR_INVALID_ARG, "bad image size %dx%d", H, W);
+    TDM_CHECK(dx > 0 && dy > 0 && dz > 0, TDM_ERR_INVALID_ARG, "bad grid %dx%dx%d", dx, dy, dz);
+    TDM_CHECK((double)dx * dy * dz < 9.0e18, TDM_ERR_INVALID_ARG, "grid too large");
+    TDM_CHECK((long)H * W < (1L << 31), TDM_ERR_INVALID_ARG, "image too large");
+    TDM_CHECK(vs > 0 && trunc > 0, TDM_ERR_INVALID_ARG, "voxel size and truncation must be > 0");
+    return TDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdm_tsdf_init(tdm_ctx *ctx, float *tsdf, float *weight, size_t n, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(tsdf && weight, TDM_ERR_INVALID_ARG, "null grid");
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    if (n == 0) return TDM_OK;
+    unsigned blocks = (unsigned)std::min<size_t>(tdm::blocks_for(n, 256), 65536);
+    tsdf_init_kernel<<<blocks, 256, 0, tdm::as_stream(stream)>>>(tsdf, weight, n);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_integrate(tdm_ctx *ctx, const float *cam_K, const float *cam2base, const float *depth_im,
+                  int im_height, int im_width, int dim_x, int dim_y, int dim_z, float origin_x,
+                  float origin_y, float origin_z, float voxel_size, float trunc_margin,
+                  float *tsdf, float *weight, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(cam_K && cam2base && depth_im && tsdf && weight, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    long plane = (long)dim_x * dim_y;
+    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRun - 1) / kZRun);
+    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
+    integrate_frame_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+        cam_K, cam2base, depth_im, im_height, im_width, dim_x, dim_y, dim_z, origin_x, origin_y,
+        origin_z, voxel_size, trunc_margin, tsdf, weight);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base,
+                         const float *depth_ims, int num_frames, int im_height, int im_width,
+                         int dim_x, int dim_y, int dim_z, float origin_x, float origin_y,
+                         float origin_z, float voxel_size, float trunc_margin, float *tsdf,
+                         float *weight, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(cam_K && cam2base && depth_ims && tsdf && weight, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_CHECK(num_frames >= 0, TDM_ERR_INVALID_ARG, "negative frame count");
+    TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    if (num_frames == 0) return TDM_OK;
+    long plane = (long)dim_x * dim_y;
+    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRun - 1) / kZRun);
+    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
+    integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+        cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z,
+        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_depth_from_mm(tdm_ctx *ctx, const uint16_t *depth_mm, size_t n, float *depth_m,
+                      void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(depth_mm && depth_m, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    if (n == 0) return TDM_OK;
+    depth_from_mm_kernel<<<tdm::blocks_for(n, 256), 256, 0, tdm::as_stream(stream)>>>(
+        depth_mm, depth_m, n);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,374 @@
+"""ctypes bindings of lib3dmatch_amd.so (the C ABI declared in include/tdm.h).
+
+Device arrays are torch tensors on a ROCm device (torch is used only for
+device memory and streams); every compute call goes to the HIP kernels in
+lib3dmatch_amd.so.  There is no CPU fallback: if the library is missing,
+``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from ctypes import POINTER, c_char_p, c_float, c_int, c_size_t, c_void_p
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib3dmatch_amd.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "tdm.h")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_IO = 2
+ERR_FORMAT = 3
+ERR_HIP = 4
+ERR_NOMEM = 5
+ERR_UNSUPPORTED = 6
+ERR_NO_DEVICE = 7
+
+PREC_FP32 = 0
+PREC_BF16 = 1
+DECODE_EXACT = 0
+DECODE_REFERENCE = 1
+
+
+class TdmError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"tdm error {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.tdm_status_string.restype = c_char_p
+        L.tdm_last_error.restype = c_char_p
+        L.tdm_net_layer_name.restype = c_char_p
+        L.tdm_free.restype = None
+        _lib = L
+    return _lib
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/tdm.h."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(tdm_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        msg = lib().tdm_last_error().decode(errors="replace")
+        raise TdmError(status, msg)
+
+
+def _p(t) -> c_void_p:
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return c_void_p(0)
+    return c_void_p(t.data_ptr())
+
+
+def _stream(stream=None) -> c_void_p:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return c_void_p(s.cuda_stream)
+
+
+def _f(x) -> c_float:
+    return c_float(float(x))
+
+
+def _int3(v):
+    return (c_int * 3)(*[int(a) for a in v])
+
+
+def _float3(v):
+    return (c_float * 3)(*[float(a) for a in np.asarray(v, dtype=np.float32)])
+
+
+# ---- TSDF -----------------------------------------------------------------------
+
+
+def tsdf_init(tsdf, weight, stream=None) -> None:
+    _check(lib().tdm_tsdf_init(None, _p(tsdf), _p(weight), c_size_t(tsdf.numel()), _stream(stream)))
+
+
+def integrate(cam_K, cam2base, depth, dims, origin, voxel_size, trunc, tsdf, weight, stream=None):
+    """One frame (Integrate, depth-fusion/demo.cu:18-68).  depth: [H, W]."""
+    H, W = depth.shape[-2:]
+    _check(lib().tdm_integrate(None, _p(cam_K), _p(cam2base), _p(depth), c_int(H), c_int(W),
+                               c_int(dims[0]), c_int(dims[1]), c_int(dims[2]), _f(origin[0]),
+                               _f(origin[1]), _f(origin[2]), _f(voxel_size), _f(trunc),
+                               _p(tsdf), _p(weight), _stream(stream)))
+
+
+def integrate_frames(cam_K, cam2base, depths, dims, origin, voxel_size, trunc, tsdf, weight,
+                     stream=None):
+    """All frames in one pass.  cam2base: [F, 16]; depths: [F, H, W]."""
+    F, H, W = depths.shape
+    _check(lib().tdm_integrate_frames(None, _p(cam_K), _p(cam2base), _p(depths), c_int(F), c_int(H),
+                                      c_int(W), c_int(dims[0]), c_int(dims[1]), c_int(dims[2]),
+                                      _f(origin[0]), _f(origin[1]), _f(origin[2]),
+                                      _f(voxel_size), _f(trunc), _p(tsdf), _p(weight),
+                                      _stream(stream)))
+
+
+def depth_from_mm(mm, out, stream=None) -> None:
+    _check(lib().tdm_depth_from_mm(None, _p(mm), c_size_t(mm.numel()), _p(out), _stream(stream)))
+
+
+# ---- TDF ------------------------------------------------------------------------
+
+
+def grid_params(pts, voxel_size=0.01, padding=15, stream=None):
+    """(dims[3] ints, origin[3] float32) for a device point tensor [N, 3]."""
+    dims = (c_int * 3)()
+    origin = (c_float * 3)()
+    _check(lib().tdm_grid_params(None, _p(pts), c_int(pts.shape[0]), _f(voxel_size),
+                                 c_int(padding), dims, origin, _stream(stream)))
+    return [int(d) for d in dims], np.array(list(origin), dtype=np.float32)
+
+
+def occupancy(pts, dims, origin, voxel_size, occ, stream=None) -> None:
+    _check(lib().tdm_occupancy(None, _p(pts), c_int(pts.shape[0]), _int3(dims), _float3(origin),
+                               _f(voxel_size), _p(occ), _stream(stream)))
+
+
+def compute_tdf(occ, tdf, dims, voxel_size, trunc, decode=DECODE_EXACT, stream=None) -> None:
+    _check(lib().tdm_compute_tdf(None, _p(occ), _p(tdf), c_int(dims[0]), c_int(dims[1]),
+                                 c_int(dims[2]), _f(voxel_size), _f(trunc), c_int(decode),
+                                 _stream(stream)))
+
+
+def tdf_from_points(pts, dims, origin, voxel_size, trunc, tdf, decode=DECODE_EXACT, stream=None):
+    _check(lib().tdm_tdf_from_points(None, _p(pts), c_int(pts.shape[0]), _int3(dims),
+                                     _float3(origin), _f(voxel_size), _f(trunc), c_int(decode),
+                                     _p(tdf), _stream(stream)))
+
+
+def keypoints_to_grid(kp: np.ndarray, origin, voxel_size) -> np.ndarray:
+    kp = np.ascontiguousarray(kp, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(kp.shape, dtype=np.int32)
+    _check(lib().tdm_keypoints_to_grid(kp.ctypes.data_as(c_void_p), c_int(kp.shape[0]),
+                                       _float3(origin), _f(voxel_size),
+                                       out.ctypes.data_as(c_void_p)))
+    return out
+
+
+def extract_patches(tdf, dims, kp_grid, out, patch_size=30, stream=None) -> None:
+    _check(lib().tdm_extract_patches(None, _p(tdf), c_int(dims[0]), c_int(dims[1]), c_int(dims[2]),
+                                     _p(kp_grid), c_int(kp_grid.shape[0]), c_int(patch_size),
+                                     _p(out), _stream(stream)))
+
+
+# ---- network --------------------------------------------------------------------
+
+
+class Net:
+    """tdm_net: Marvin JSON network + .marvin weights on one GPU."""
+
+    def __init__(self, json_path: str, precision: int = PREC_FP32, max_batch: int = 0):
+        self._h = c_void_p()
+        _check(lib().tdm_net_create(None, json_path.encode(), c_int(precision), c_int(max_batch),
+                                    ctypes.byref(self._h)))
+        self.precision = precision
+
+    def close(self) -> None:
+        if self._h:
+            lib().tdm_net_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def output_dim(self) -> int:
+        return lib().tdm_net_output_dim(self._h)
+
+    @property
+    def layer_names(self) -> list[str]:
+        L = lib()
+        return [L.tdm_net_layer_name(self._h, i).decode() for i in range(L.tdm_net_num_layers(self._h))]
+
+    def load_weights(self, marvin_path: str) -> None:
+        _check(lib().tdm_net_load_weights(self._h, marvin_path.encode()))
+
+    def set_layer(self, name: str, weight: np.ndarray, bias: np.ndarray) -> None:
+        w = np.ascontiguousarray(weight, dtype=np.float32)
+        b = np.ascontiguousarray(bias, dtype=np.float32)
+        _check(lib().tdm_net_set_layer(self._h, name.encode(), w.ctypes.data_as(c_void_p),
+                                       c_size_t(w.size), b.ctypes.data_as(c_void_p),
+                                       c_size_t(b.size)))
+
+    def forward(self, data, feat, stream=None) -> None:
+        """data: [n, 1, P, P, P] float device tensor; feat: [n, D] float."""
+        _check(lib().tdm_net_forward(self._h, _p(data), c_int(data.shape[0]), _p(feat),
+                                     _stream(stream)))
+
+    def forward_grid(self, tdf, dims, kp_grid, feat, stream=None) -> None:
+        _check(lib().tdm_net_forward_grid(self._h, _p(tdf), c_int(dims[0]), c_int(dims[1]),
+                                          c_int(dims[2]), _p(kp_grid), c_int(kp_grid.shape[0]),
+                                          _p(feat), _stream(stream)))
+
+    def forward_layer(self, data, layer_index: int, out, stream=None) -> None:
+        _check(lib().tdm_net_forward_layer(self._h, _p(data), c_int(data.shape[0]),
+                                           c_int(layer_index), _p(out), c_size_t(out.numel()),
+                                           _stream(stream)))
+
+
+# ---- formats (host) ----------------------------------------------------------------
+
+
+def _take_floats(ptr, count) -> np.ndarray:
+    arr = np.ctypeslib.as_array(ctypes.cast(ptr, POINTER(c_float)), shape=(count,)).copy()
+    lib().tdm_free(ptr)
+    return arr
+
+
+def read_ply_points(path: str) -> np.ndarray:
+    ptr, n = c_void_p(), c_int()
+    _check(lib().tdm_read_ply_points(path.encode(), ctypes.byref(ptr), ctypes.byref(n)))
+    return _take_floats(ptr, n.value * 3).reshape(-1, 3)
+
+
+def write_ply_points(path: str, pts: np.ndarray) -> None:
+    pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    _check(lib().tdm_write_ply_points(path.encode(), pts.ctypes.data_as(c_void_p), c_int(len(pts))))
+
+
+def write_keypts_bin(path: str, kp: np.ndarray) -> None:
+    kp = np.ascontiguousarray(kp, dtype=np.float32).reshape(-1, 3)
+    _check(lib().tdm_write_keypts_bin(path.encode(), kp.ctypes.data_as(c_void_p), c_int(len(kp))))
+
+
+def read_keypts_bin(path: str) -> np.ndarray:
+    ptr, n = c_void_p(), c_int()
+    _check(lib().tdm_read_keypts_bin(path.encode(), ctypes.byref(ptr), ctypes.byref(n)))
+    return _take_floats(ptr, n.value * 3).reshape(-1, 3)
+
+
+def write_desc_bin(path: str, desc: np.ndarray) -> None:
+    desc = np.ascontiguousarray(desc, dtype=np.float32)
+    _check(lib().tdm_write_desc_bin(path.encode(), desc.ctypes.data_as(c_void_p),
+                                    c_int(desc.shape[0]), c_int(desc.shape[1])))
+
+
+def read_desc_bin(path: str) -> np.ndarray:
+    ptr, n, d = c_void_p(), c_int(), c_int()
+    _check(lib().tdm_read_desc_bin(path.encode(), ctypes.byref(ptr), ctypes.byref(n), ctypes.byref(d)))
+    return _take_floats(ptr, n.value * d.value).reshape(n.value, d.value)
+
+
+def write_grid_bin(path: str, grid: np.ndarray, origin, voxel_size, trunc) -> None:
+    """grid: [dz, dy, dx] float32 (x fastest)."""
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    dims = (g.shape[2], g.shape[1], g.shape[0])
+    _check(lib().tdm_write_grid_bin(path.encode(), _int3(dims), _float3(origin), _f(voxel_size),
+                                    _f(trunc), g.ctypes.data_as(c_void_p)))
+
+
+def read_grid_bin(path: str):
+    dims = (c_int * 3)()
+    origin = (c_float * 3)()
+    vs, tr, ptr = c_float(), c_float(), c_void_p()
+    _check(lib().tdm_read_grid_bin(path.encode(), dims, origin, ctypes.byref(vs), ctypes.byref(tr),
+                                   ctypes.byref(ptr)))
+    dx, dy, dz = list(dims)
+    grid = _take_floats(ptr, dx * dy * dz).reshape(dz, dy, dx)
+    return grid, np.array(list(origin), dtype=np.float32), vs.value, tr.value
+
+
+def read_patch_volumes(path: str) -> np.ndarray:
+    ptr, n = c_void_p(), c_int()
+    dims = (c_int * 3)()
+    _check(lib().tdm_read_patch_volumes(path.encode(), ctypes.byref(ptr), ctypes.byref(n), dims))
+    d = list(dims)
+    return _take_floats(ptr, n.value * d[0] * d[1] * d[2]).reshape(n.value, d[2], d[1], d[0])
+
+
+def write_feat_bin(path: str, feat: np.ndarray) -> None:
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    _check(lib().tdm_write_feat_bin(path.encode(), feat.ctypes.data_as(c_void_p),
+                                    c_int(feat.shape[0]), c_int(feat.shape[1])))
+
+
+def load_matrix(path: str, rows: int, cols: int) -> np.ndarray:
+    out = np.zeros(rows * cols, dtype=np.float32)
+    _check(lib().tdm_load_matrix(path.encode(), c_int(rows), c_int(cols), out.ctypes.data_as(c_void_p)))
+    return out.reshape(rows, cols)
+
+
+def read_depth_png(path: str, height: int = 480, width: int = 640) -> np.ndarray:
+    out = np.zeros(height * width, dtype=np.float32)
+    _check(lib().tdm_read_depth_png(path.encode(), c_int(height), c_int(width),
+                                    out.ctypes.data_as(c_void_p)))
+    return out.reshape(height, width)
+
+
+def read_png_u16(path: str) -> np.ndarray:
+    ptr, h, w = c_void_p(), c_int(), c_int()
+    _check(lib().tdm_read_png_u16(path.encode(), ctypes.byref(ptr), ctypes.byref(h), ctypes.byref(w)))
+    arr = np.ctypeslib.as_array(ctypes.cast(ptr, POINTER(ctypes.c_uint16)),
+                                shape=(h.value * w.value,)).copy()
+    lib().tdm_free(ptr)
+    return arr.reshape(h.value, w.value)
+
+
+def multiply_matrix(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(16)
+    b = np.ascontiguousarray(b, dtype=np.float32).reshape(16)
+    out = np.zeros(16, dtype=np.float32)
+    _check(lib().tdm_multiply_matrix(a.ctypes.data_as(c_void_p), b.ctypes.data_as(c_void_p),
+                                     out.ctypes.data_as(c_void_p)))
+    return out.reshape(4, 4)
+
+
+def invert_matrix(m: np.ndarray) -> np.ndarray:
+    m = np.ascontiguousarray(m, dtype=np.float32).reshape(16)
+    out = np.zeros(16, dtype=np.float32)
+    _check(lib().tdm_invert_matrix(m.ctypes.data_as(c_void_p), out.ctypes.data_as(c_void_p)))
+    return out.reshape(4, 4)
+
+
+def save_surface_ply(path: str, tsdf: np.ndarray, weight: np.ndarray, origin, voxel_size,
+                     tsdf_thresh=0.2, weight_thresh=1.0) -> int:
+    t = np.ascontiguousarray(tsdf, dtype=np.float32)
+    w = np.ascontiguousarray(weight, dtype=np.float32)
+    dims = (t.shape[2], t.shape[1], t.shape[0])
+    n = c_int()
+    _check(lib().tdm_save_surface_ply(path.encode(), _int3(dims), _f(voxel_size), _float3(origin),
+                                      t.ctypes.data_as(c_void_p), w.ctypes.data_as(c_void_p),
+                                      _f(tsdf_thresh), _f(weight_thresh), ctypes.byref(n)))
+    return n.value
+
+
+def marvin_write(path: str, tensors: list[tuple[str, np.ndarray]]) -> None:
+    """Float tensors as a .marvin file (name, array) in order."""
+    n = len(tensors)
+    names = (c_char_p * n)(*[name.encode() for name, _ in tensors])
+    arrays = [np.ascontiguousarray(a, dtype=np.float32) for _, a in tensors]
+    ndims = (c_int * n)(*[a.ndim for a in arrays])
+    dim_arrays = [(c_int * max(a.ndim, 1))(*a.shape) for a in arrays]
+    dims = (POINTER(c_int) * n)(*[ctypes.cast(d, POINTER(c_int)) for d in dim_arrays])
+    data = (c_void_p * n)(*[a.ctypes.data for a in arrays])
+    _check(lib().tdm_marvin_write(path.encode(), c_int(n), names, ndims, dims, data))
+
+
+def marvin_count(path: str):
+    nt, nv = c_int(), c_size_t()
+    _check(lib().tdm_marvin_count(path.encode(), ctypes.byref(nt), ctypes.byref(nv)))
+    return nt.value, nv.value

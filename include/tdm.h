@@ -1,0 +1,253 @@
+/*
+ * tdm.h -- C ABI of lib3dmatch_amd.so, the MI355X (gfx950) implementation of
+ * the 3DMatch-toolbox hot path: TSDF depth fusion, point cloud -> TDF, patch
+ * extraction and the 3DMatch 30^3 -> 512 descriptor network.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (TDM_OK == 0).  Nothing calls
+ *    exit(); tdm_last_error() gives a message for the calling thread.
+ *  - Pointers documented as "device" are HIP device pointers on the device
+ *    the context belongs to; "host" pointers are ordinary host memory.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Work is enqueued on it; entry points that must read a result back on
+ *    the host (marked "synchronizes") wait for the stream.
+ *  - Grids are float32, x fastest: index = (z*dim_y + y)*dim_x + x
+ *    (depth-fusion/demo.cu:61, core/demo.cu:47).
+ *  - A tdm_ctx owns per-device scratch memory.  One context per host thread;
+ *    ctx may be NULL in every call that takes one, meaning a library-owned
+ *    context for the current HIP device.
+ *
+ * Every compute entry point is backed by hand-written HIP kernels; there is
+ * no CPU fallback.  Without a usable GPU they return TDM_ERR_NO_DEVICE.
+ */
+#ifndef TDM_H
+#define TDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDM_VERSION 1
+
+enum tdm_status {
+    TDM_OK = 0,
+    TDM_ERR_INVALID_ARG = 1,
+    TDM_ERR_IO = 2,
+    TDM_ERR_FORMAT = 3,
+    TDM_ERR_HIP = 4,
+    TDM_ERR_NOMEM = 5,
+    TDM_ERR_UNSUPPORTED = 6,
+    TDM_ERR_NO_DEVICE = 7,
+};
+
+/* Descriptor-network arithmetic. */
+enum tdm_precision {
+    TDM_PREC_FP32 = 0, /* float storage, float MFMA (reference DATATYPE 1)      */
+    TDM_PREC_BF16 = 1, /* bf16 weights/activations, fp32 accumulation on MFMA   */
+};
+
+/* How ComputeTDF turns a linear voxel index into (x, y, z). */
+enum tdm_decode {
+    TDM_DECODE_EXACT = 0,     /* integer decode                              */
+    TDM_DECODE_REFERENCE = 1, /* float32 decode of core/demo.cu:32-34, which
+                                 differs from EXACT only on grids > 2^24 voxels */
+};
+
+typedef struct tdm_ctx tdm_ctx;
+typedef struct tdm_net tdm_net;
+
+/* ---- library ------------------------------------------------------------ */
+int tdm_version(void);
+const char *tdm_status_string(int status);
+const char *tdm_last_error(void);
+/* Frees memory returned by tdm_read_* helpers. */
+void tdm_free(void *p);
+
+/* Context on HIP device `device` (replaces the demos' implicit default
+ * device, core/marvin.hpp:7435 cudaSetDevice). */
+int tdm_ctx_create(int device, tdm_ctx **out);
+int tdm_ctx_destroy(tdm_ctx *ctx);
+
+/* ---- TSDF fusion ---------------------------------------------------------- */
+
+/* Fills tsdf with 1 and weight with 0 (depth-fusion/demo.cu:131-135).
+ * tsdf, weight: device, n floats each. */
+int tdm_tsdf_init(tdm_ctx *ctx, float *tsdf, float *weight, size_t n, void *stream);
+
+/* One depth frame into the grid; replaces the Integrate kernel and its launch
+ * (depth-fusion/demo.cu:18-68, 179-182).  Same arguments and meaning:
+ *   cam_K     device, 9 floats, row-major intrinsics
+ *   cam2base  device, 16 floats, row-major camera-to-base pose
+ *   depth_im  device, im_height*im_width floats, metres (0 = invalid)
+ *   tsdf, weight  device grids of dim_x*dim_y*dim_z floats, updated in place.
+ * Results are bit-identical to the reference arithmetic (no FMA contraction,
+ * IEEE division and square root).  Unlike the reference launch (one thread
+ * per y, so dim_y <= 1024) any grid size is accepted. */
+int tdm_integrate(tdm_ctx *ctx, const float *cam_K, const float *cam2base, const float *depth_im,
+                  int im_height, int im_width, int dim_x, int dim_y, int dim_z, float origin_x,
+                  float origin_y, float origin_z, float voxel_size, float trunc_margin,
+                  float *tsdf, float *weight, void *stream);
+
+/* num_frames frames in one pass over the grid: cam2base is num_frames*16
+ * floats, depth_ims num_frames*im_height*im_width floats (both device).
+ * Each voxel applies the frames in order, so the grids are bit-identical to
+ * num_frames calls of tdm_integrate (the frame loop of
+ * depth-fusion/demo.cu:156-183); each voxel is read and written once. */
+int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base,
+                         const float *depth_ims, int num_frames, int im_height, int im_width,
+                         int dim_x, int dim_y, int dim_z, float origin_x, float origin_y,
+                         float origin_z, float voxel_size, float trunc_margin, float *tsdf,
+                         float *weight, void *stream);
+
+/* u16 millimetre depth (device, n values) -> metres with values above 6 m set
+ * to 0 (depth-fusion/utils.hpp:74-86 after the PNG decode). */
+int tdm_depth_from_mm(tdm_ctx *ctx, const uint16_t *depth_mm, size_t n, float *depth_m,
+                      void *stream);
+
+/* ---- point cloud -> TDF ---------------------------------------------------- */
+
+/* Grid size and origin for a cloud (core/demo.cu:97-122):
+ * dims = round(extent/voxel_size) + 1 + 2*padding, origin = min - padding*vs
+ * + vs/2.  pts: device, num_pts*3 floats.  Synchronizes `stream`. */
+int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_size, int padding,
+                    int dims_out[3], float origin_out[3], void *stream);
+
+/* Float occupancy grid (1 where a point rounds to the voxel, else 0), the
+ * input of ComputeTDF (core/demo.cu:127-135).  occ: device, dims product. */
+int tdm_occupancy(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                  const float origin[3], float voxel_size, float *occ, void *stream);
+
+/* ComputeTDF over the whole grid (core/demo.cu:22-57 and its launch loop
+ * :151-158): tdf = 1 on occupied voxels, else max(0, 1 - d/r) over occupied
+ * voxels of the (2r+1)^3 cube, r = round(trunc_margin/voxel_size), d the
+ * voxel-centre distance.  occ, tdf: device.  Bit-exact (values come from a
+ * table of 1 - sqrtf(d^2)/r built on the host).  decode: enum tdm_decode.
+ * The origin is not used by the reference kernel and is not taken here. */
+int tdm_compute_tdf(tdm_ctx *ctx, const float *occ, float *tdf, int dim_x, int dim_y, int dim_z,
+                    float voxel_size, float trunc_margin, int decode, void *stream);
+
+/* Points straight to TDF, with no float occupancy grid: what
+ * core/demo.cu:127-158 computes, for dims/origin from tdm_grid_params.
+ * pts, tdf: device. */
+int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                        const float origin[3], float voxel_size, float trunc_margin, int decode,
+                        float *tdf, void *stream);
+
+/* Keypoint grid coordinates round((p - origin)/voxel_size) (core/demo.cu:
+ * 174-176).  kp: host num*3 floats; grid_out: host num*3 ints. */
+int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
+                          int *grid_out);
+
+/* 30^3 (patch_size^3) TDF patches around keypoints, z-major, x fastest,
+ * spanning k-15 .. k+14 (core/demo.cu:195-218).  Voxels outside the grid read
+ * as 0 (the reference reads out of bounds there).  tdf: device grid;
+ * kp_grid: device num*3 ints; patches: device num*patch_size^3 floats. */
+int tdm_extract_patches(tdm_ctx *ctx, const float *tdf, int dim_x, int dim_y, int dim_z,
+                        const int *kp_grid, int num, int patch_size, float *patches, void *stream);
+
+/* ---- descriptor network ------------------------------------------------------ */
+
+/* Builds the net of a Marvin JSON file (core/marvin.hpp:7530-7542 Net(json)
+ * + Malloc, :7632-7651) for the given precision.  Supported layers:
+ * PlaceHolderData (data dims from "dim", [N,1,30,30,30]), Convolution (3-D,
+ * stride 1, no padding), Activation ReLU, Pooling max.  max_batch bounds the
+ * patches of one internal pass (0 = default 256). */
+int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_batch,
+                   tdm_net **out);
+/* Reads a .marvin file and sets <layer>.weight / <layer>.bias by name
+ * (core/marvin.hpp:7605-7615, 3321-3377).  Float, half and double tensors
+ * are accepted and converted.  A tensor whose element count does not match
+ * is skipped, as the reference does; a layer left without weights makes
+ * tdm_net_forward fail (the reference runs on uninitialized memory). */
+int tdm_net_load_weights(tdm_net *net, const char *marvin_path);
+/* Sets one layer's weights from host float arrays: weight [O][I][k][k][k],
+ * bias [O]. */
+int tdm_net_set_layer(tdm_net *net, const char *layer_name, const float *weight,
+                      size_t weight_count, const float *bias, size_t bias_count);
+int tdm_net_destroy(tdm_net *net);
+/* Descriptor width (num_output of the last layer, 512). */
+int tdm_net_output_dim(const tdm_net *net);
+/* Number of convolution layers and their names, in order. */
+int tdm_net_num_layers(const tdm_net *net);
+const char *tdm_net_layer_name(const tdm_net *net, int index);
+
+/* Net::forward (core/marvin.hpp:7653-7679) on `num` patches.
+ * data: device, num*patch^3 floats ([num,1,30,30,30] as the "data"
+ * response); feat: device, num*output_dim floats (the "feat" response).
+ * Any num >= 1 (the reference needs multiples of the JSON batch size). */
+int tdm_net_forward(tdm_net *net, const float *data, int num, float *feat, void *stream);
+
+/* Patch gather fused into the first convolution: descriptors straight from
+ * a TDF grid and keypoint grid coordinates (core/demo.cu:195-231 without the
+ * host gather and the per-batch copies).  tdf, kp_grid, feat: device. */
+int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, int dim_z,
+                         const int *kp_grid, int num, float *feat, void *stream);
+
+/* Output of convolution layer `layer_index` (after its ReLU, and after the
+ * pool that follows it, if any) for `num` patches, as float NCDHW into out
+ * (device).  For per-layer error measurements. */
+int tdm_net_forward_layer(tdm_net *net, const float *data, int num, int layer_index, float *out,
+                          size_t out_count, void *stream);
+
+/* ---- file formats (host) --------------------------------------------------- */
+
+/* Binary PLY as read by core/demo.cu:69-93: 7 header lines, the vertex count
+ * on line 3, then num*3 little-endian float32.  *pts is freed with tdm_free. */
+int tdm_read_ply_points(const char *path, float **pts, int *num);
+/* Writes that layout (the header of depth-fusion/utils.hpp:33-41). */
+int tdm_write_ply_points(const char *path, const float *pts, int num);
+
+/* <prefix>.keypts.bin: float N then N*3 floats (core/demo.cu:238-244). */
+int tdm_write_keypts_bin(const char *path, const float *kp, int num);
+int tdm_read_keypts_bin(const char *path, float **kp, int *num);
+/* <prefix>.desc.3dmatch.bin: float N, float D, then N*D floats
+ * (core/demo.cu:248-255). */
+int tdm_write_desc_bin(const char *path, const float *desc, int num, int dim);
+int tdm_read_desc_bin(const char *path, float **desc, int *num, int *dim);
+
+/* tsdf.bin / tdf.bin: 8 floats [dx, dy, dz, ox, oy, oz, voxel_size,
+ * trunc_margin] then the grid (depth-fusion/demo.cu:196-213,
+ * core/demo.cu:257-274). */
+int tdm_write_grid_bin(const char *path, const int dims[3], const float origin[3],
+                       float voxel_size, float trunc_margin, const float *grid);
+int tdm_read_grid_bin(const char *path, int dims[3], float origin[3], float *voxel_size,
+                      float *trunc_margin, float **grid);
+
+/* Patch volume file of get3DMatchDescriptors (evaluation/keypoint-matching/
+ * get3DMatchDescriptors.cu:11-24): float N, 3 float dims, N*prod(dims)
+ * floats.  And feat.bin (:35-64): float N, 4 float dims, N*prod floats. */
+int tdm_read_patch_volumes(const char *path, float **data, int *num, int dims[3]);
+int tdm_write_feat_bin(const char *path, const float *feat, int num, int dim);
+
+/* M x N whitespace-separated floats (depth-fusion/utils.hpp:60-70). */
+int tdm_load_matrix(const char *path, int rows, int cols, float *out);
+/* 16-bit grayscale PNG, millimetres -> metres, >6 m -> 0
+ * (depth-fusion/utils.hpp:74-86).  depth: host, height*width floats. */
+int tdm_read_depth_png(const char *path, int height, int width, float *depth);
+/* Raw u16 values of a 16-bit grayscale PNG. *mm freed with tdm_free. */
+int tdm_read_png_u16(const char *path, uint16_t **mm, int *height, int *width);
+/* depth-fusion/utils.hpp:89-109 and :112-238 (float arithmetic kept). */
+int tdm_multiply_matrix(const float a[16], const float b[16], float out[16]);
+int tdm_invert_matrix(const float m[16], float out[16]);
+/* Surface points |tsdf| < tsdf_thresh && weight > weight_thresh as a binary
+ * PLY at origin + idx*voxel_size (depth-fusion/utils.hpp:14-56).  Host grids.
+ * Returns the point count in *num_written when non-NULL. */
+int tdm_save_surface_ply(const char *path, const int dims[3], float voxel_size,
+                         const float origin[3], const float *tsdf, const float *weight,
+                         float tsdf_thresh, float weight_thresh, int *num_written);
+
+/* .marvin tensor files (core/marvin.hpp:2581-2611, 2764-2798): a sequence of
+ * {u8 type, u32 sizeof, i32 name_len, name, i32 ndims, i32 dims[], data}.
+ * Writes float tensors. */
+int tdm_marvin_write(const char *path, int num_tensors, const char *const *names,
+                     const int *ndims, const int *const *dims, const float *const *data);
+/* Number of tensors and total float elements in a .marvin file. */
+int tdm_marvin_count(const char *path, int *num_tensors, size_t *num_values);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDM_H */

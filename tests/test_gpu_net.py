@@ -1,0 +1,136 @@
+"""GPU parity of the descriptor network (Net::forward, core/marvin.hpp:7653-7679)
+against the FP64 oracle (torch CPU float64 conv3d/relu/max_pool3d).
+
+Tolerances (relative L2 over each layer's whole output, NCDHW):
+  fp32 path  <= 1e-5 per layer and for the descriptors;
+  bf16 path  <= 3e-2 per layer, <= 3e-2 for the descriptors.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import tdm
+from conftest import rel_l2
+from tdm import netdef, synth
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+BF16_TOL = 3e-2
+
+
+@pytest.fixture(scope="module")
+def net_files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("net")
+    js = str(d / "net.json")
+    wt = str(d / "weights.marvin")
+    netdef.write_net_json(js)
+    tensors = netdef.he_normal_weights(5)
+    # non-zero biases exercise the bias epilogue
+    rng = np.random.default_rng(9)
+    tensors = [(n, a if n.endswith(".weight") else rng.normal(0, 0.05, a.shape).astype(np.float32))
+               for n, a in tensors]
+    tdm.marvin_write(wt, tensors)
+    return js, wt, dict(tensors)
+
+
+@pytest.fixture(scope="module")
+def patches():
+    """TDF patches around surface keypoints of a synthetic fragment."""
+    cloud = synth.fragment(60_000, seed=21)
+    vs = np.float32(0.01)
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    tdf = oracle.compute_tdf(occ, vs, np.float32(0.05))
+    idx = synth.keypoint_indices(len(cloud), 6, seed=22)
+    kg = oracle.keypoint_grid(cloud[idx], origin, vs)
+    return oracle.extract_patches(tdf, kg), tdf, dims, kg
+
+
+@pytest.fixture(scope="module")
+def fp64(net_files, patches):
+    _, _, w = net_files
+    return oracle.net_forward_fp64(patches[0], w, netdef.layer_shapes())
+
+
+def _run_layers(torch, net, p):
+    shapes = netdef.layer_shapes()
+    data = torch.from_numpy(p).to("cuda:0")
+    outs = []
+    for i, (_n, _ci, co, _k, _d, od, pooled) in enumerate(shapes):
+        e = pooled if pooled is not None else od
+        out = torch.empty(len(p), co, e, e, e, dtype=torch.float32, device="cuda:0")
+        net.forward_layer(data, i, out)
+        outs.append(out.cpu().numpy())
+    return outs
+
+
+@pytest.mark.parametrize("prec,tol", [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL)])
+def test_per_layer_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
+    js, wt, _ = net_files
+    net = tdm.Net(js, prec, max_batch=4)
+    net.load_weights(wt)
+    outs = _run_layers(torch_gpu, net, patches[0])
+    errs = [rel_l2(o, r) for o, r in zip(outs, fp64)]
+    print("per-layer rel-L2:", ["%.2e" % e for e in errs], flush=True)
+    for name, e in zip(net.layer_names, errs):
+        assert e <= tol, f"{name}: rel-L2 {e:.3e} > {tol}"
+
+
+@pytest.mark.parametrize("prec,tol", [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL)])
+def test_descriptors_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
+    js, wt, _ = net_files
+    net = tdm.Net(js, prec)
+    net.load_weights(wt)
+    p = patches[0]
+    data = torch_gpu.from_numpy(p).to("cuda:0")
+    feat = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
+    net.forward(data, feat)
+    got = feat.cpu().numpy()
+    e = rel_l2(got, fp64[-1].reshape(len(p), 512))
+    print(f"descriptor rel-L2 {e:.3e}", flush=True)
+    assert e <= tol
+
+
+def test_grid_path_equals_patch_path(torch_gpu, net_files, patches):
+    """Patch gather fused into conv1 gives bit-identical descriptors."""
+    js, wt, _ = net_files
+    p, tdf, dims, kg = patches
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16):
+        net = tdm.Net(js, prec)
+        net.load_weights(wt)
+        a = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
+        b = torch_gpu.empty_like(a)
+        net.forward(torch_gpu.from_numpy(p).to("cuda:0"), a)
+        net.forward_grid(torch_gpu.from_numpy(tdf).to("cuda:0"), dims,
+                         torch_gpu.from_numpy(kg).to("cuda:0"), b)
+        assert torch_gpu.equal(a, b)
+
+
+def test_batch_invariance(torch_gpu, net_files, patches):
+    """A patch's descriptor does not depend on the other patches of its pass."""
+    js, wt, _ = net_files
+    p = patches[0]
+    net = tdm.Net(js, tdm.PREC_FP32, max_batch=4)
+    net.load_weights(wt)
+    full = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
+    net.forward(torch_gpu.from_numpy(p).to("cuda:0"), full)
+    one = torch_gpu.empty(1, 512, dtype=torch_gpu.float32, device="cuda:0")
+    for i in range(len(p)):
+        net.forward(torch_gpu.from_numpy(p[i:i + 1]).to("cuda:0"), one)
+        assert torch_gpu.equal(one[0], full[i])
+
+
+def test_fp32_vs_oracle_fp32_loop(torch_gpu, net_files, patches):
+    """Against the float32 C loop nest (a different summation order): the gap is
+    summation-order noise, far below the fp64 tolerance."""
+    js, wt, w = net_files
+    p = patches[0][:2]
+    ref = oracle.net_forward_fp32(p, w, netdef.layer_shapes()).reshape(2, 512)
+    net = tdm.Net(js, tdm.PREC_FP32)
+    net.load_weights(wt)
+    feat = torch_gpu.empty(2, 512, dtype=torch_gpu.float32, device="cuda:0")
+    net.forward(torch_gpu.from_numpy(p).to("cuda:0"), feat)
+    assert rel_l2(feat.cpu().numpy(), ref) <= FP32_TOL

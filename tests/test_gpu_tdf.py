@@ -1,0 +1,137 @@
+"""GPU parity of point cloud -> TDF (core/demo.cu:22-158) and the patch gather
+(core/demo.cu:195-218) against the oracle.  Integer work (grid dims, voxel
+coordinates, keypoint coordinates) and the TDF values are bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+import tdm
+from tdm import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def cloud():
+    return synth.fragment(30_000, seed=11)
+
+
+def test_grid_params_match(torch_gpu, cloud):
+    for vs in (np.float32(0.01), np.float32(0.04), np.float32(0.013)):
+        dims, origin = tdm.grid_params(_dev(torch_gpu, cloud), vs, 15)
+        rdims, rorigin = oracle.grid_params(cloud, vs, 15)
+        assert dims == rdims
+        assert np.array_equal(origin.view(np.uint32), rorigin.view(np.uint32))
+
+
+@pytest.mark.parametrize("decode", [tdm.DECODE_EXACT, tdm.DECODE_REFERENCE])
+def test_tdf_from_points_bit_exact(torch_gpu, cloud, decode):
+    vs = np.float32(0.04)
+    trunc = np.float32(vs * np.float32(5))
+    pts = _dev(torch_gpu, cloud)
+    dims, origin = tdm.grid_params(pts, vs, 15)
+    print(f"case tdf_from_points dims={dims} decode={decode}", flush=True)
+    tdf = torch_gpu.empty(dims[2], dims[1], dims[0], dtype=torch_gpu.float32, device="cuda:0")
+    tdm.tdf_from_points(pts, dims, origin, vs, trunc, tdf, decode)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    ref = oracle.compute_tdf(occ, vs, trunc, "reference" if decode else "exact")
+    got = tdf.cpu().numpy()
+    assert np.count_nonzero(got == 1.0) == np.count_nonzero(occ)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), \
+        f"{np.count_nonzero(got != ref)} voxels differ"
+
+
+def test_occupancy_and_compute_tdf(torch_gpu, cloud):
+    vs = np.float32(0.05)
+    trunc = np.float32(vs * np.float32(5))
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    occ_d = torch_gpu.empty(dims[2], dims[1], dims[0], dtype=torch_gpu.float32, device="cuda:0")
+    tdm.occupancy(_dev(torch_gpu, cloud), dims, origin, vs, occ_d)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    assert np.array_equal(occ_d.cpu().numpy(), occ)
+    tdf = torch_gpu.empty_like(occ_d)
+    tdm.compute_tdf(occ_d, tdf, dims, vs, trunc)
+    ref = oracle.compute_tdf(occ, vs, trunc)
+    assert np.array_equal(tdf.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 6, 8])
+def test_search_radius_variants(torch_gpu, r):
+    rng = np.random.default_rng(100 + r)
+    dims = (131, 45, 38)  # x spans several 64-voxel tiles and a partial word
+    occ = (rng.random((dims[2], dims[1], dims[0])) < 0.002).astype(np.float32)
+    vs = np.float32(0.01)
+    trunc = np.float32(vs * np.float32(r))
+    assert int(round(float(trunc / vs))) == r
+    occ_d = _dev(torch_gpu, occ)
+    tdf = torch_gpu.empty_like(occ_d)
+    tdm.compute_tdf(occ_d, tdf, dims, vs, trunc)
+    ref = oracle.compute_tdf(occ, vs, trunc)
+    assert np.array_equal(tdf.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_empty_and_single_point(torch_gpu):
+    dims = (40, 30, 20)
+    occ = np.zeros((20, 30, 40), np.float32)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    occ_d = _dev(torch_gpu, occ)
+    tdf = torch_gpu.full_like(occ_d, 7.0)
+    tdm.compute_tdf(occ_d, tdf, dims, vs, trunc)
+    assert np.all(tdf.cpu().numpy() == 0)
+    occ[0, 0, 0] = 1  # a corner point: the clipped cube
+    occ[19, 29, 39] = 1
+    tdm.compute_tdf(_dev(torch_gpu, occ), tdf, dims, vs, trunc)
+    assert np.array_equal(tdf.cpu().numpy(), oracle.compute_tdf(occ, vs, trunc))
+
+
+def test_reference_decode_beyond_2_24(torch_gpu):
+    """On a grid of more than 2^24 voxels the reference's float32 decode of
+    voxel_idx (core/demo.cu:32-34) is wrong for part of the voxels; the
+    reference decode mode reproduces its output there."""
+    dims = (300, 300, 190)  # 17.1 M voxels
+    n = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(7)
+    occ = (rng.random((dims[2], dims[1], dims[0])) < 0.0005).astype(np.float32)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    occ_d = _dev(torch_gpu, occ)
+    ref_mode = torch_gpu.empty_like(occ_d)
+    exact_mode = torch_gpu.empty_like(occ_d)
+    print("case refdecode 300x300x190", flush=True)
+    tdm.compute_tdf(occ_d, ref_mode, dims, vs, trunc, tdm.DECODE_REFERENCE)
+    tdm.compute_tdf(occ_d, exact_mode, dims, vs, trunc, tdm.DECODE_EXACT)
+    got_ref = ref_mode.cpu().numpy().reshape(-1)
+    got_exact = exact_mode.cpu().numpy().reshape(-1)
+    x, y, z = oracle.float_decode_all(n, dims[0], dims[1])
+    v = np.arange(n)
+    bad = (x != v % dims[0]) | (y != (v // dims[0]) % dims[1]) | (z != v // (dims[0] * dims[1]))
+    assert bad.sum() > 0
+    # where the decode is right both modes agree
+    assert np.array_equal(got_ref[~bad], got_exact[~bad])
+    # every mis-decoded voxel plus a random sample, against the oracle
+    idx = np.concatenate([np.nonzero(bad)[0], rng.choice(n, 20000, replace=False)])
+    ref = oracle.compute_tdf_ref_at(occ, idx, vs, trunc)
+    assert np.array_equal(got_ref[idx].view(np.uint32), ref.view(np.uint32))
+
+
+def test_extract_patches_bit_exact(torch_gpu, cloud):
+    vs = np.float32(0.02)
+    trunc = np.float32(vs * np.float32(5))
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    tdf = oracle.compute_tdf(occ, vs, trunc)
+    idx = synth.keypoint_indices(len(cloud), 64, seed=12)
+    kp = cloud[idx]
+    kg = tdm.keypoints_to_grid(kp, origin, vs)
+    assert np.array_equal(kg, oracle.keypoint_grid(kp, origin, vs))
+    # keypoints near and beyond the border read zeros outside the grid
+    kg = np.concatenate([kg, np.array([[0, 0, 0], [dims[0] - 1, dims[1] - 1, dims[2] - 1], [14, 15, 16]], np.int32)])
+    out = torch_gpu.empty(len(kg), 30, 30, 30, dtype=torch_gpu.float32, device="cuda:0")
+    tdm.extract_patches(_dev(torch_gpu, tdf), dims, _dev(torch_gpu, kg), out)
+    assert np.array_equal(out.cpu().numpy(), oracle.extract_patches(tdf, kg))
