@@ -9,6 +9,7 @@
 // two ping-pong device buffers sized for `max_batch` patches, NDHWC, in the
 // storage type of the precision (float or bf16); the last layer writes float
 // descriptors straight into the caller's buffer.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -54,8 +55,17 @@ struct tdm_net {
     size_t act_bytes = 0;
     float *partial = nullptr;
     size_t partial_bytes = 0;
-    void *pool_tmp = nullptr;
-    size_t pool_tmp_bytes = 0;
+    // Per-layer timing (tdm_net_set_profiling): events recorded on the stream
+    // around each layer's launches, resolved when statistics are read.
+    bool profiling = false;
+    struct Timed {
+        int layer;
+        int patches;
+        hipEvent_t start, stop;
+    };
+    std::vector<Timed> pending;
+    std::vector<double> layer_ms;
+    std::vector<long> layer_launches, layer_patches;
 };
 
 namespace {
@@ -281,11 +291,36 @@ int choose_splits(const tdm_net &net, const Layer &l, size_t nq_chunks) {
 // Runs layers [0, upto] for `n` patches of one pass.  The input of the first
 // layer comes from `src`.  When upto is the last layer, its float output goes
 // to `final_out`; otherwise the output stays in *result (storage type).
+int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *final_out,
+              int last, int &cur, hipStream_t s);
+
 int run_pass(tdm_net &net, const tdm::VolumeSource &src, int n, int upto, float *final_out,
              const void **result, hipStream_t s) {
     int cur = 0;
     const int last = (int)net.layers.size() - 1;
     for (int li = 0; li <= upto; ++li) {
+        tdm_net::Timed timed{li, n, nullptr, nullptr};
+        if (net.profiling) {
+            TDM_HIP(hipEventCreate(&timed.start));
+            TDM_HIP(hipEventCreate(&timed.stop));
+            TDM_HIP(hipEventRecord(timed.start, s));
+        }
+        const int st = run_layer(net, li, src, n, final_out, last, cur, s);
+        if (net.profiling) {
+            TDM_HIP(hipEventRecord(timed.stop, s));
+            net.pending.push_back(timed);
+        }
+        TDM_TRY(st);
+    }
+    if (result) *result = net.act[cur];
+    return TDM_OK;
+}
+
+// One layer of run_pass; `cur` is the ping-pong buffer holding its input and
+// is advanced past its output.
+int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *final_out,
+              int last, int &cur, hipStream_t s) {
+    {
         Layer &l = net.layers[li];
         const bool to_final = li == last && final_out;
         void *out = to_final ? static_cast<void *>(final_out) : net.act[1 - cur];
@@ -299,7 +334,7 @@ int run_pass(tdm_net &net, const tdm::VolumeSource &src, int n, int upto, float 
                 TDM_TRY(tdm::launch_maxpool(net.dt, net.act[cur], n, l.cout, l.out_d, net.act[1 - cur], s));
                 cur = 1 - cur;
             }
-            continue;
+            return TDM_OK;
         }
         tdm::ConvDesc d;
         d.C = l.cin;
@@ -322,7 +357,25 @@ int run_pass(tdm_net &net, const tdm::VolumeSource &src, int n, int upto, float 
             cur = 1 - cur;
         }
     }
-    if (result) *result = net.act[cur];
+    return TDM_OK;
+}
+
+int resolve_timing(tdm_net &net) {
+    const size_t L = net.layers.size();
+    net.layer_ms.resize(L, 0.0);
+    net.layer_launches.resize(L, 0);
+    net.layer_patches.resize(L, 0);
+    for (auto &t : net.pending) {
+        float ms = 0.0f;
+        TDM_HIP(hipEventSynchronize(t.stop));
+        TDM_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+        net.layer_ms[t.layer] += ms;
+        net.layer_launches[t.layer] += 1;
+        net.layer_patches[t.layer] += t.patches;
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    net.pending.clear();
     return TDM_OK;
 }
 
@@ -433,6 +486,32 @@ int tdm_net_set_layer(tdm_net *net, const char *layer_name, const float *weight,
         if (net->layers[li].name == layer_name)
             return set_layer(net, net->layers[li], li, weight, weight_count, bias, bias_count);
     return tdm::fail(TDM_ERR_INVALID_ARG, "no layer named %s", layer_name);
+}
+
+int tdm_net_set_profiling(tdm_net *net, int enable) {
+    tdm::clear_error();
+    TDM_CHECK(net, TDM_ERR_INVALID_ARG, "null net");
+    TDM_HIP(hipSetDevice(net->device));
+    TDM_TRY(resolve_timing(*net));
+    net->profiling = enable != 0;
+    std::fill(net->layer_ms.begin(), net->layer_ms.end(), 0.0);
+    std::fill(net->layer_launches.begin(), net->layer_launches.end(), 0);
+    std::fill(net->layer_patches.begin(), net->layer_patches.end(), 0);
+    return TDM_OK;
+}
+
+int tdm_net_layer_stats(tdm_net *net, int layer_index, double *total_ms, long *launches,
+                        long *patches) {
+    tdm::clear_error();
+    TDM_CHECK(net && total_ms && launches && patches, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(layer_index >= 0 && layer_index < (int)net->layers.size(), TDM_ERR_INVALID_ARG,
+              "layer index %d out of range", layer_index);
+    TDM_HIP(hipSetDevice(net->device));
+    TDM_TRY(resolve_timing(*net));
+    *total_ms = net->layer_ms[layer_index];
+    *launches = net->layer_launches[layer_index];
+    *patches = net->layer_patches[layer_index];
+    return TDM_OK;
 }
 
 int tdm_net_output_dim(const tdm_net *net) { return net && !net->layers.empty() ? net->layers.back().cout : 0; }

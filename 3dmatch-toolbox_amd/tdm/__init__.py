@@ -214,6 +214,16 @@ class Net:
                                        c_size_t(w.size), b.ctypes.data_as(c_void_p),
                                        c_size_t(b.size)))
 
+    def set_profiling(self, enable: bool) -> None:
+        _check(lib().tdm_net_set_profiling(self._h, c_int(1 if enable else 0)))
+
+    def layer_stats(self, index: int):
+        """(total ms, passes, patches) of layer `index` since profiling was enabled."""
+        ms, n, p = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().tdm_net_layer_stats(self._h, c_int(index), ctypes.byref(ms), ctypes.byref(n),
+                                         ctypes.byref(p)))
+        return ms.value, n.value, p.value
+
     def forward(self, data, feat, stream=None) -> None:
         """data: [n, 1, P, P, P] float device tensor; feat: [n, D] float."""
         _check(lib().tdm_net_forward(self._h, _p(data), c_int(data.shape[0]), _p(feat),

@@ -192,6 +192,16 @@ int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, i
 int tdm_net_forward_layer(tdm_net *net, const float *data, int num, int layer_index, float *out,
                           size_t out_count, void *stream);
 
+/* Per-layer timing, the counterpart of Marvin's debug_mode tic/toc per layer
+ * (core/marvin.hpp:437-456, 7656-7676): while enabled, every forward records
+ * HIP events on its stream around each layer's kernels.  Enabling or
+ * disabling resets the statistics. */
+int tdm_net_set_profiling(tdm_net *net, int enable);
+/* Accumulated device time (ms), number of timed passes and patches of one
+ * layer since profiling was enabled.  Waits for the recorded events. */
+int tdm_net_layer_stats(tdm_net *net, int layer_index, double *total_ms, long *launches,
+                        long *patches);
+
 /* ---- file formats (host) --------------------------------------------------- */
 
 /* Binary PLY as read by core/demo.cu:69-93: 7 header lines, the vertex count

@@ -376,3 +376,16 @@ ORC_API void orc_compute_tdf_ref_at(const float *occ, int dx, int dy, int dz, fl
         out[i] = occ[v] > 0 ? 1.0f : tdf_search(occ, dx, dy, dz, gx, gy, gz, r);
     }
 }
+
+/* orc_compute_tdf_exact over the voxel index range [v0, v1) only (for a
+ * bounded CPU-baseline sample); tdf is the whole grid. */
+ORC_API void orc_compute_tdf_exact_range(const float *occ, float *tdf, int dx, int dy, int dz,
+                                         float voxel_size, float trunc_margin, long v0, long v1)
+{
+    int r = (int)roundf(trunc_margin / voxel_size);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (long v = v0; v < v1; ++v) {
+        int gx = (int)(v % dx), gy = (int)((v / dx) % dy), gz = (int)(v / ((long)dx * dy));
+        tdf[v] = occ[v] > 0 ? 1.0f : tdf_search(occ, dx, dy, dz, gx, gy, gz, r);
+    }
+}

@@ -223,3 +223,13 @@ def float_decode_all(n: int, dx: int, dy: int):
     x = (v - (zf * fdx * fdy) - (yf * fdx)).astype(np.float32)
     x = np.trunc(x).astype(np.int64)
     return x, y, z
+
+
+def compute_tdf_range(occ: np.ndarray, tdf: np.ndarray, v0: int, v1: int, voxel_size=0.01,
+                      trunc=0.05) -> None:
+    """Exact-decode TDF of voxels [v0, v1) into tdf (whole grid, in place)."""
+    occ = _f32(occ)
+    assert tdf.dtype == np.float32 and tdf.flags.c_contiguous and tdf.shape == occ.shape
+    dz, dy, dx = occ.shape
+    lib().orc_compute_tdf_exact_range(_ptr(occ), _ptr(tdf), c_int(dx), c_int(dy), c_int(dz),
+                                      c_float(voxel_size), c_float(trunc), c_long(v0), c_long(v1))
