@@ -35,6 +35,7 @@ struct Layer {
     int pooled_d = 0;
     std::vector<float> w, b;  // host copies (float, reference layout)
     bool has_w = false, has_b = false;
+    bool dirty = false;      // host weights newer than the device copy
     void *w_dev = nullptr;   // packed [N][taps][C] (conv1: [N][KP])
     float *b_dev = nullptr;
     int final_d() const { return pool ? pooled_d : out_d; }
@@ -379,27 +380,48 @@ int resolve_timing(tdm_net &net) {
     return TDM_OK;
 }
 
+// The device is bound at the first forward (a net can be built and given
+// weights without a GPU); later calls make it current again.
+int bind_device(tdm_net *net) {
+    if (net->device < 0) {
+        tdm_ctx *c = net->ctx;
+        TDM_TRY(tdm::resolve_ctx(c, &c));
+        net->ctx = c;
+        net->device = c->device;
+    }
+    TDM_HIP(hipSetDevice(net->device));
+    return TDM_OK;
+}
+
 int check_ready(tdm_net *net) {
     TDM_CHECK(net, TDM_ERR_INVALID_ARG, "null net");
     for (const Layer &l : net->layers)
         TDM_CHECK(l.has_w && l.has_b, TDM_ERR_INVALID_ARG, "layer %s has no weights loaded",
                   l.name.c_str());
-    TDM_HIP(hipSetDevice(net->device));
+    TDM_TRY(bind_device(net));
+    for (size_t li = 0; li < net->layers.size(); ++li)
+        if (net->layers[li].dirty) {
+            // Work queued earlier may still read the old weights.
+            TDM_HIP(hipDeviceSynchronize());
+            TDM_TRY(upload_layer(*net, net->layers[li], li));
+            net->layers[li].dirty = false;
+        }
     TDM_TRY(ensure_buffers(*net));
     return TDM_OK;
 }
 
-int set_layer(tdm_net *net, Layer &l, size_t li, const float *w, size_t wn, const float *b,
-              size_t bn) {
+// Host copy of one layer's weights; uploaded (packed) at the next forward.
+int set_layer(tdm_net *net, Layer &l, const float *w, size_t wn, const float *b, size_t bn) {
     const size_t want_w = (size_t)l.cout * l.cin * l.k * l.k * l.k;
     TDM_CHECK(wn == want_w && bn == (size_t)l.cout, TDM_ERR_INVALID_ARG,
               "%s: expected %zu weights and %d biases, got %zu and %zu", l.name.c_str(), want_w,
               l.cout, wn, bn);
+    (void)net;
     l.w.assign(w, w + wn);
     l.b.assign(b, b + bn);
     l.has_w = l.has_b = true;
-    TDM_HIP(hipSetDevice(net->device));
-    return upload_layer(*net, l, li);
+    l.dirty = true;
+    return TDM_OK;
 }
 
 }  // namespace
@@ -413,14 +435,13 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
     TDM_CHECK(precision == TDM_PREC_FP32 || precision == TDM_PREC_BF16, TDM_ERR_INVALID_ARG,
               "unknown precision %d", precision);
     TDM_CHECK(max_batch >= 0 && max_batch <= 65536, TDM_ERR_INVALID_ARG, "bad max_batch");
-    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     std::string text, err;
     TDM_TRY(read_text(json_path, text));
     tdm::Json root;
     TDM_CHECK(tdm::json_parse(text, root, err), TDM_ERR_FORMAT, "%s: %s", json_path, err.c_str());
     tdm_net *net = new tdm_net();
     net->ctx = ctx;
-    net->device = ctx->device;
+    net->device = ctx ? ctx->device : -1;
     net->dt = precision == TDM_PREC_FP32 ? tdm::DType::F32 : tdm::DType::BF16;
     net->max_batch = max_batch ? max_batch : 256;
     int s = build_layers(root, *net);
@@ -434,8 +455,11 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
 
 int tdm_net_destroy(tdm_net *net) {
     if (!net) return TDM_OK;
-    (void)hipSetDevice(net->device);
-    (void)hipDeviceSynchronize();
+    if (net->device >= 0) {
+        (void)hipSetDevice(net->device);
+        (void)hipDeviceSynchronize();
+        (void)resolve_timing(*net);
+    }
     for (Layer &l : net->layers) {
         if (l.w_dev) (void)hipFree(l.w_dev);
         if (l.b_dev) (void)hipFree(l.b_dev);
@@ -467,12 +491,12 @@ int tdm_net_load_weights(tdm_net *net, const char *marvin_path) {
                 else std::fprintf(stderr, "[Warning] %s.bias is found but not loaded because the numels are mismatched\n", l.name.c_str());
             }
         }
-        if (w && b) TDM_TRY(set_layer(net, l, li, w->data.data(), w->data.size(), b->data.data(), b->data.size()));
+        if (w && b) TDM_TRY(set_layer(net, l, w->data.data(), w->data.size(), b->data.data(), b->data.size()));
         else if (w || b) {
             // Keep the other half if it was set earlier.
             std::vector<float> wv = w ? w->data : l.w, bv = b ? b->data : l.b;
             if (wv.size() == want_w && bv.size() == (size_t)l.cout)
-                TDM_TRY(set_layer(net, l, li, wv.data(), wv.size(), bv.data(), bv.size()));
+                TDM_TRY(set_layer(net, l, wv.data(), wv.size(), bv.data(), bv.size()));
         }
     }
     return TDM_OK;
@@ -484,15 +508,20 @@ int tdm_net_set_layer(tdm_net *net, const char *layer_name, const float *weight,
     TDM_CHECK(net && layer_name && weight && bias, TDM_ERR_INVALID_ARG, "null pointer argument");
     for (size_t li = 0; li < net->layers.size(); ++li)
         if (net->layers[li].name == layer_name)
-            return set_layer(net, net->layers[li], li, weight, weight_count, bias, bias_count);
+            return set_layer(net, net->layers[li], weight, weight_count, bias, bias_count);
     return tdm::fail(TDM_ERR_INVALID_ARG, "no layer named %s", layer_name);
 }
 
 int tdm_net_set_profiling(tdm_net *net, int enable) {
     tdm::clear_error();
     TDM_CHECK(net, TDM_ERR_INVALID_ARG, "null net");
-    TDM_HIP(hipSetDevice(net->device));
-    TDM_TRY(resolve_timing(*net));
+    if (net->device >= 0) {
+        TDM_HIP(hipSetDevice(net->device));
+        TDM_TRY(resolve_timing(*net));
+    }
+    net->layer_ms.resize(net->layers.size());
+    net->layer_launches.resize(net->layers.size());
+    net->layer_patches.resize(net->layers.size());
     net->profiling = enable != 0;
     std::fill(net->layer_ms.begin(), net->layer_ms.end(), 0.0);
     std::fill(net->layer_launches.begin(), net->layer_launches.end(), 0);
@@ -506,8 +535,13 @@ int tdm_net_layer_stats(tdm_net *net, int layer_index, double *total_ms, long *l
     TDM_CHECK(net && total_ms && launches && patches, TDM_ERR_INVALID_ARG, "null pointer argument");
     TDM_CHECK(layer_index >= 0 && layer_index < (int)net->layers.size(), TDM_ERR_INVALID_ARG,
               "layer index %d out of range", layer_index);
-    TDM_HIP(hipSetDevice(net->device));
-    TDM_TRY(resolve_timing(*net));
+    if (net->device >= 0) {
+        TDM_HIP(hipSetDevice(net->device));
+        TDM_TRY(resolve_timing(*net));
+    }
+    net->layer_ms.resize(net->layers.size());
+    net->layer_launches.resize(net->layers.size());
+    net->layer_patches.resize(net->layers.size());
     *total_ms = net->layer_ms[layer_index];
     *launches = net->layer_launches[layer_index];
     *patches = net->layer_patches[layer_index];
@@ -517,6 +551,17 @@ int tdm_net_layer_stats(tdm_net *net, int layer_index, double *total_ms, long *l
 int tdm_net_output_dim(const tdm_net *net) { return net && !net->layers.empty() ? net->layers.back().cout : 0; }
 
 int tdm_net_num_layers(const tdm_net *net) { return net ? (int)net->layers.size() : 0; }
+
+int tdm_net_layer_info(const tdm_net *net, int index, int info[8]) {
+    tdm::clear_error();
+    TDM_CHECK(net && info, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(index >= 0 && index < (int)net->layers.size(), TDM_ERR_INVALID_ARG,
+              "layer index %d out of range", index);
+    const Layer &l = net->layers[index];
+    const int v[8] = {l.cin, l.cout, l.k, l.in_d, l.out_d, l.relu ? 1 : 0, l.pool ? 1 : 0, l.final_d()};
+    std::memcpy(info, v, sizeof v);
+    return TDM_OK;
+}
 
 const char *tdm_net_layer_name(const tdm_net *net, int index) {
     if (!net || index < 0 || index >= (int)net->layers.size()) return nullptr;

@@ -83,12 +83,27 @@ __global__ void bbox_partial_kernel(const float *__restrict__ pts, int n, float 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// One block of 256 threads folds the per-block partials (min/max are exact,
+// so the order does not matter).
 __global__ void bbox_final_kernel(const float *__restrict__ part, int nblocks, float *__restrict__ out) {
-    int a = threadIdx.x;
-    if (a >= 6) return;
-    float v = part[a];
-    for (int b = 1; b < nblocks; ++b) v = a < 3 ? fminf(v, part[b * 6 + a]) : fmaxf(v, part[b * 6 + a]);
-    out[a] = v;
+    __shared__ float s[6][256];
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+            v[a] = a < 3 ? fminf(v[a], part[b * 6 + a]) : fmaxf(v[a], part[b * 6 + a]);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) s[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+                s[a][threadIdx.x] = a < 3 ? fminf(s[a][threadIdx.x], s[a][threadIdx.x + off])
+                                          : fmaxf(s[a][threadIdx.x], s[a][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // ---- occupancy ---------------------------------------------------------------------
@@ -161,10 +176,10 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *_
                                                                int dx, int dy, int dz, Lut lut,
                                                                float *__restrict__ tdf) {
     constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
+    __shared__ unsigned sw[HZ * HY * 4];        // halo bit words, x in [X0-32, X0+96)
     __shared__ unsigned char s1[HZ * HY * kTX];  // pass x, all halo rows
     __shared__ unsigned char s2[HZ * kTY * kTX]; // pass y
     __shared__ float slut[kLutSize];
-    __shared__ int any_bits;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -172,25 +187,28 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *_
     const long rw = row_words(dx);
     const long wbase = X0 >> 5;  // first word of the tile in a row
 
-    if (threadIdx.x == 0) any_bits = 0;
     for (int i = threadIdx.x; i < kLutSize; i += kTdfThreads) slut[i] = lut.v[i];
-    __syncthreads();
+    // All halo words at once (independent loads, so their latencies overlap).
+    unsigned seen = 0;
+    for (int i = threadIdx.x; i < HZ * HY * 4; i += kTdfThreads) {
+        const int row = i >> 2, wi = (int)(i & 3);
+        const int y = Y0 + row % HY - R, z = Z0 + row / HY - R;
+        const long w = wbase - 1 + wi;
+        unsigned v = 0u;
+        if (y >= 0 && y < dy && z >= 0 && z < dz && w >= 0 && w < rw)
+            v = bits[((long)z * dy + y) * rw + w];
+        sw[i] = v;
+        seen |= v;
+    }
+    const bool empty = __syncthreads_or(seen != 0u) == 0;
 
     // Pass x over every halo row: nearest set bit within R along x.
-    bool seen = false;
-    for (int row = wave; row < HY * HZ; row += kTdfThreads / 64) {
-        const int yy = row % HY, zz = row / HY;
-        const int y = Y0 + yy - R, z = Z0 + zz - R;
+    for (int row = wave; !empty && row < HY * HZ; row += kTdfThreads / 64) {
         unsigned char f = kNone;
-        if (y >= 0 && y < dy && z >= 0 && z < dz) {
-            const unsigned *r = bits + ((long)z * dy + y) * rw;
-            // Words covering x in [X0-32, X0+96).
-            unsigned w0 = wbase - 1 >= 0 ? r[wbase - 1] : 0u;
-            unsigned w1 = wbase < rw ? r[wbase] : 0u;
-            unsigned w2 = wbase + 1 < rw ? r[wbase + 1] : 0u;
-            unsigned w3 = wbase + 2 < rw ? r[wbase + 2] : 0u;
+        {
+            const unsigned w0 = sw[row * 4 + 0], w1 = sw[row * 4 + 1];
+            const unsigned w2 = sw[row * 4 + 2], w3 = sw[row * 4 + 3];
             if (w0 | w1 | w2 | w3) {
-                seen = true;
                 unsigned long long lo = ((unsigned long long)w1 << 32) | w0;
                 unsigned long long hi = ((unsigned long long)w3 << 32) | w2;
                 // Window of 2R+1 bits starting at local bit lane + 32 - R.
@@ -212,10 +230,8 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *_
         }
         s1[row * kTX + lane] = f;
     }
-    if (seen) any_bits = 1;  // benign race: every writer stores 1
     __syncthreads();
 
-    const bool empty = any_bits == 0;
     if (!empty) {
         // Pass y: rows (zz in halo, y in tile).
         for (int row = wave; row < HZ * kTY; row += kTdfThreads / 64) {
@@ -393,7 +409,7 @@ int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_siz
     float *res = part + 6 * nblocks;
     bbox_partial_kernel<<<nblocks, 256, 0, s>>>(pts, num_pts, part);
     TDM_LAUNCHED();
-    bbox_final_kernel<<<1, 64, 0, s>>>(part, nblocks, res);
+    bbox_final_kernel<<<1, 256, 0, s>>>(part, nblocks, res);
     TDM_LAUNCHED();
     float box[6];
     TDM_HIP(hipMemcpyAsync(box, res, sizeof box, hipMemcpyDeviceToHost, s));

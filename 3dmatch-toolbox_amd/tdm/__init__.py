@@ -204,6 +204,12 @@ class Net:
         L = lib()
         return [L.tdm_net_layer_name(self._h, i).decode() for i in range(L.tdm_net_num_layers(self._h))]
 
+    def layer_info(self, index: int) -> dict:
+        info = (c_int * 8)()
+        _check(lib().tdm_net_layer_info(self._h, c_int(index), info))
+        keys = ("cin", "cout", "window", "in_extent", "out_extent", "relu", "pool", "final_extent")
+        return dict(zip(keys, list(info)))
+
     def load_weights(self, marvin_path: str) -> None:
         _check(lib().tdm_net_load_weights(self._h, marvin_path.encode()))
 

@@ -154,7 +154,9 @@ int tdm_extract_patches(tdm_ctx *ctx, const float *tdf, int dim_x, int dim_y, in
  * + Malloc, :7632-7651) for the given precision.  Supported layers:
  * PlaceHolderData (data dims from "dim", [N,1,30,30,30]), Convolution (3-D,
  * stride 1, no padding), Activation ReLU, Pooling max.  max_batch bounds the
- * patches of one internal pass (0 = default 256). */
+ * patches of one internal pass (0 = default 256).  Building the net and
+ * setting weights need no GPU; the device (ctx's, or the current one for a
+ * NULL ctx) is bound, and weights are uploaded, at the first forward. */
 int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_batch,
                    tdm_net **out);
 /* Reads a .marvin file and sets <layer>.weight / <layer>.bias by name
@@ -173,6 +175,11 @@ int tdm_net_output_dim(const tdm_net *net);
 /* Number of convolution layers and their names, in order. */
 int tdm_net_num_layers(const tdm_net *net);
 const char *tdm_net_layer_name(const tdm_net *net, int index);
+/* Shape of convolution layer `index`: info[0..7] = input channels, output
+ * channels, window, input extent, output extent, ReLU (0/1), max pool after
+ * it (0/1), extent after the pool.  The counterpart of the response shapes
+ * Net::Malloc prints (core/marvin.hpp:2988-2996). */
+int tdm_net_layer_info(const tdm_net *net, int index, int info[8]);
 
 /* Net::forward (core/marvin.hpp:7653-7679) on `num` patches.
  * data: device, num*patch^3 floats ([num,1,30,30,30] as the "data"

@@ -1,0 +1,117 @@
+"""Pins the oracle's TDF pipeline (oracle.c, core/demo.cu:22-57, 97-135,
+164-218) against library routines and brute force:
+  - exact-decode TDF == table[EDT^2] with scipy.ndimage.distance_transform_edt,
+  - pure-Python brute force of the 11^3 search on a tiny grid,
+  - the float32 decode against an independent numpy float32 emulation, with
+    the mis-decode counts (0 at 256^3, 5 111 392 at 300^3),
+  - occupancy / keypoint coordinates against numpy float32 rounding,
+  - patch gather against numpy slicing with zero padding."""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle
+from tdm import synth
+
+
+def lut(r):
+    d2 = np.arange(3 * 8 * 8 + 1, dtype=np.float32)
+    v = np.float32(1) - np.sqrt(d2) / np.float32(r)
+    return np.where(v > 0, v, np.float32(0)).astype(np.float32)
+
+
+def edt_tdf(occ, r):
+    d = ndimage.distance_transform_edt(occ == 0)
+    d2 = np.rint(d * d).astype(np.int64)
+    out = np.zeros(occ.shape, np.float32)
+    near = d2 < r * r
+    out[near] = lut(r)[d2[near]]
+    return out
+
+
+@pytest.mark.parametrize("r", [2, 5])
+def test_exact_tdf_equals_truncated_edt(r):
+    rng = np.random.default_rng(r)
+    occ = (rng.random((40, 50, 70)) < 0.003).astype(np.float32)
+    vs = np.float32(0.01)
+    trunc = np.float32(vs * np.float32(r))
+    got = oracle.compute_tdf(occ, vs, trunc)
+    assert np.array_equal(got, edt_tdf(occ, r))
+    assert np.array_equal(got[occ > 0], np.ones(int(occ.sum()), np.float32))
+
+
+def test_cloud_tdf_equals_truncated_edt():
+    cloud = synth.fragment(20_000, seed=5)
+    vs = np.float32(0.04)
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    assert np.array_equal(oracle.compute_tdf(occ, vs, np.float32(0.2)), edt_tdf(occ, 5))
+
+
+def test_brute_force_tiny_grid():
+    rng = np.random.default_rng(9)
+    occ = (rng.random((6, 7, 9)) < 0.05).astype(np.float32)
+    r = 3
+    dz, dy, dx = occ.shape
+    want = np.zeros_like(occ)
+    pts = np.argwhere(occ > 0)
+    for z in range(dz):
+        for y in range(dy):
+            for x in range(dx):
+                if occ[z, y, x] > 0:
+                    want[z, y, x] = 1
+                    continue
+                best = np.float32(0)
+                for (iz, iy, ix) in pts:
+                    if abs(iz - z) <= r and abs(iy - y) <= r and abs(ix - x) <= r:
+                        v = np.float32(1) - np.sqrt(np.float32((x - ix) ** 2 + (y - iy) ** 2 + (z - iz) ** 2)) / np.float32(r)
+                        best = max(best, v)
+                want[z, y, x] = best
+    got = oracle.compute_tdf(occ, np.float32(0.01), np.float32(0.03))
+    assert np.array_equal(got, want)
+    assert np.array_equal(oracle.compute_tdf(occ, np.float32(0.01), np.float32(0.03), "reference"), want)
+
+
+def test_float_decode_counts():
+    for dims, bad_want in (((256, 256, 256), 0), ((300, 300, 300), 5_111_392)):
+        n = dims[0] * dims[1] * dims[2]
+        x, y, z = oracle.float_decode_all(n, dims[0], dims[1])
+        v = np.arange(n)
+        bad = (x != v % dims[0]) | (y != (v // dims[0]) % dims[1]) | (z != v // (dims[0] * dims[1]))
+        assert int(bad.sum()) == bad_want
+        # the C restatement agrees with the numpy emulation on a sample
+        idx = np.random.default_rng(0).choice(n, 2000, replace=False)
+        if bad_want:
+            idx = np.concatenate([idx, np.nonzero(bad)[0][:2000]])
+        for i in idx:
+            assert oracle.float_decode(int(i), dims[0], dims[1]) == (x[i], y[i], z[i])
+
+
+def test_occupancy_and_keypoint_rounding():
+    cloud = synth.fragment(5000, seed=6)
+    vs = np.float32(0.01)
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    lo = cloud.min(0)
+    hi = cloud.max(0)
+    want_dims = [int(np.round(np.float32(hi[a] - lo[a]) / vs)) + 1 + 30 for a in range(3)]
+    assert dims == want_dims
+    assert np.array_equal(origin, (lo - np.float32(15) * vs + vs / np.float32(2)).astype(np.float32))
+    # round() of the reference is half away from zero; coordinates here are >= 0
+    q = (cloud - origin) / vs
+    g = np.where(q - np.floor(q) == np.float32(0.5), np.floor(q) + 1, np.rint(q)).astype(np.int64)
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    want = np.zeros_like(occ)
+    want[g[:, 2], g[:, 1], g[:, 0]] = 1
+    assert np.array_equal(occ, want)
+    assert np.array_equal(oracle.keypoint_grid(cloud[:100], origin, vs), g[:100])
+
+
+def test_patch_gather_zero_padded():
+    rng = np.random.default_rng(8)
+    tdf = rng.random((40, 35, 50), dtype=np.float32)
+    kg = np.array([[25, 17, 20], [0, 0, 0], [49, 34, 39], [14, 15, 16]], np.int32)
+    got = oracle.extract_patches(tdf, kg)
+    pad = np.pad(tdf, 30)
+    for k, p in zip(kg, got):
+        x, y, z = k + 30
+        assert np.array_equal(p, pad[z - 15:z + 15, y - 15:y + 15, x - 15:x + 15])
