@@ -24,6 +24,7 @@
 // aligned group of 8 rows is one pool window (rows 4g..4g+3 of a 16-row block
 // sit in one lane; the partner group is 16 lanes away).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
@@ -56,16 +57,19 @@ __device__ inline float comp(const uint4 &v, int s) {
 template <typename T>
 struct Mfma;
 
+// One 16-byte fragment per lane for each of IB row blocks and JB column
+// blocks: fp32 runs 4 k-steps (one per dword), bf16 one 32-k step.
 template <>
 struct Mfma<float> {
-    __device__ static inline void chunk(floatx4 (&acc)[4][4], const uint4 (&a)[4],
-                                        const uint4 (&b)[4]) {
+    template <int IB, int JB>
+    __device__ static inline void chunk(floatx4 (&acc)[IB][JB], const uint4 (&a)[IB],
+                                        const uint4 (&b)[JB]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < IB; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < JB; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(comp(a[i], s), comp(b[j], s),
                                                                      acc[i][j], 0, 0, 0);
     }
@@ -73,12 +77,13 @@ struct Mfma<float> {
 
 template <>
 struct Mfma<bf16_t> {
-    __device__ static inline void chunk(floatx4 (&acc)[4][4], const uint4 (&a)[4],
-                                        const uint4 (&b)[4]) {
+    template <int IB, int JB>
+    __device__ static inline void chunk(floatx4 (&acc)[IB][JB], const uint4 (&a)[IB],
+                                        const uint4 (&b)[JB]) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < IB; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < JB; ++j) {
                 bf16x8 av, bv;
                 __builtin_memcpy(&av, &a[i], 16);
                 __builtin_memcpy(&bv, &b[j], 16);
@@ -118,6 +123,24 @@ struct ConvArgs {
     int relu;
 };
 
+// Workgroup -> tile order that keeps neighbouring tiles on one XCD.  The
+// dispatcher hands workgroup i (linear id, x fastest) to XCD i % 8, so
+// consecutive tiles -- which share input voxels through the 3x3x3 halo --
+// would land in eight different L2s.  Here XCD x gets one contiguous run of
+// the tile sequence instead.
+constexpr unsigned kXcds = 8;
+__device__ inline void xcd_tile(unsigned &tx, unsigned &ty, unsigned &tz) {
+    const unsigned nx = gridDim.x, ny = gridDim.y;
+    const unsigned total = nx * ny * gridDim.z;
+    const unsigned bid = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const unsigned per = total / kXcds, rem = total % kXcds;
+    const unsigned x = bid % kXcds, idx = bid / kXcds;
+    const unsigned t = x * per + min(x, rem) + idx;
+    tx = t % nx;
+    ty = (t / nx) % ny;
+    tz = t / (nx * ny);
+}
+
 // Input voxel index of output row m: rows are (b, z, y, x) in order, or, with
 // POOL, (b, pz, py, px, window) with the 2x2x2 window innermost.
 template <bool POOL>
@@ -154,8 +177,10 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
     constexpr int WAVES_M = 4 / WAVES_N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / WAVES_N, wn = wave % WAVES_N;
-    const long m0 = ((long)blockIdx.x * WAVES_M + wm) * 64;
-    const int n0 = (blockIdx.y * WAVES_N + wn) * 64;
+    unsigned tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const long m0 = ((long)tx * WAVES_M + wm) * 64;
+    const int n0 = (ty * WAVES_N + wn) * 64;
     if (m0 >= p.M) return;  // no barriers below
     const int r16 = lane & 15, g = lane >> 4;
     const int lanek = g * (KC / 4);
@@ -176,8 +201,8 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
     const int nq = p.taps * cpt;
     int q0 = 0, q1 = nq;
     if (SPLIT) {
-        q0 = (int)((long)nq * blockIdx.z / p.splits);
-        q1 = (int)((long)nq * (blockIdx.z + 1) / p.splits);
+        q0 = (int)((long)nq * tz / p.splits);
+        q1 = (int)((long)nq * (tz + 1) / p.splits);
     }
     const int D = p.D, C = p.C, ks = p.ksize;
     auto offsets = [&](int q, long &aoff, long &boff) {
@@ -232,7 +257,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
 
     // Epilogue.  Lane holds rows m0+16i+4g+r (r = 0..3) of column n0+16j+r16.
     if (SPLIT) {
-        float *__restrict__ part = p.partial + (long)blockIdx.z * p.M * p.N;
+        float *__restrict__ part = p.partial + (long)tz * p.M * p.N;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -273,6 +298,173 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
             }
         }
     }
+}
+
+// Epilogue of a wave tile of IB x JB 16x16 blocks at (m0, n0): the lane holds
+// rows m0+16i+4g+r (r = 0..3) of column n0+16j+r16.  Split-K writes float
+// partials; otherwise bias, ReLU, the fused 2x2x2 pool and the store.
+template <typename OT, int IB, int JB, bool POOL, bool SPLIT>
+__device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArgs &p, long m0,
+                                     int n0, int lane, unsigned split) {
+    const int r16 = lane & 15, g = lane >> 4;
+    if (SPLIT) {
+        float *__restrict__ part = p.partial + (long)split * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+#pragma unroll
+            for (int j = 0; j < JB; ++j) {
+                const int col = n0 + 16 * j + r16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) part[row * p.N + col] = acc[i][j][r];
+                }
+            }
+        return;
+    }
+    OT *__restrict__ out = static_cast<OT *>(p.out);
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int col = n0 + 16 * j + r16;
+        const float bv = p.bias[col];
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = acc[i][j][r] + bv;
+                if (p.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+            }
+            if (POOL) {
+                float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                const long row = m0 + 16 * i + 4 * g;
+                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, mx);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) store<OT>(out + row * p.N + col, v[r]);
+                }
+            }
+        }
+    }
+}
+
+// LDS-staged implicit GEMM.  A workgroup computes a 128 x BN tile with four
+// waves in a 2 x 2 grid (wave tile 64 x BN/2).  Each stage is 128 bytes of k
+// per row -- one tap and 32 (fp32) or 64 (bf16) input channels: the 256
+// threads copy the stage's 128 + BN rows from global memory into registers
+// one stage ahead, then into the other half of a double-buffered LDS tile
+// (rows padded to 144 bytes against bank conflicts); the waves read their
+// fragments from LDS, so each A row is fetched once for BN/2-column halves
+// of two waves and each weight row once for two row halves.
+constexpr int kStageBytes = 128;
+constexpr int kLdsRow = 144;
+
+template <typename T, typename OT, int BN, bool POOL, bool SPLIT>
+__global__ __launch_bounds__(256, 2) void conv_lds_kernel(ConvArgs p) {
+    constexpr int BM = 128;
+    constexpr int WTN = BN / 2;
+    constexpr int IB = 4, JB = WTN / 16;
+    constexpr int SK = kStageBytes / (int)sizeof(T);  // k per stage
+    constexpr int A_PT = BM * 8 / 256;                // 16-byte pieces per thread
+    constexpr int B_PT = BN * 8 / 256;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[2][(BM + BN) * kLdsRow];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    unsigned tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const long mb = (long)tx * BM;
+    const int nb = ty * BN;
+    const int C = p.C, D = p.D, ks = p.ksize;
+    const long K = (long)p.taps * C;
+    const unsigned char *__restrict__ in = static_cast<const unsigned char *>(p.in);
+    const unsigned char *__restrict__ w = static_cast<const unsigned char *>(p.w);
+
+    const unsigned char *a_src[A_PT];
+    int a_dst[A_PT];
+#pragma unroll
+    for (int j = 0; j < A_PT; ++j) {
+        const int idx = tid + 256 * j, row = idx >> 3, pc = idx & 7;
+        const long m = min(mb + row, p.M - 1);
+        a_src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + pc * 16;
+        a_dst[j] = row * kLdsRow + pc * 16;
+    }
+    const unsigned char *b_src[B_PT];
+    int b_dst[B_PT];
+#pragma unroll
+    for (int j = 0; j < B_PT; ++j) {
+        const int idx = tid + 256 * j, col = idx >> 3, pc = idx & 7;
+        b_src[j] = w + ((long)(nb + col) * K) * (long)sizeof(T) + pc * 16;
+        b_dst[j] = (BM + col) * kLdsRow + pc * 16;
+    }
+
+    const int cps = C / SK;  // stages per tap
+    const int ns = p.taps * cps;
+    int s0 = 0, s1 = ns;
+    if (SPLIT) {
+        s0 = (int)((long)ns * tz / p.splits);
+        s1 = (int)((long)ns * (tz + 1) / p.splits);
+    }
+    uint4 ra[A_PT], rb[B_PT];
+    auto fetch = [&](int s) {
+        const int tap = s / cps;
+        const int c0 = (s - tap * cps) * SK;
+        int kz = 0, ky = 0, kx = 0;
+        if (ks == 3) {
+            kz = tap / 9;
+            ky = (tap / 3) % 3;
+            kx = tap % 3;
+        }
+        const long aoff = ((long)((kz * D + ky) * D + kx) * C + c0) * (long)sizeof(T);
+        const long boff = ((long)tap * C + c0) * (long)sizeof(T);
+#pragma unroll
+        for (int j = 0; j < A_PT; ++j) ra[j] = ld16(a_src[j] + aoff);
+#pragma unroll
+        for (int j = 0; j < B_PT; ++j) rb[j] = ld16(b_src[j] + boff);
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < A_PT; ++j) *reinterpret_cast<uint4 *>(&lds[buf][a_dst[j]]) = ra[j];
+#pragma unroll
+        for (int j = 0; j < B_PT; ++j) *reinterpret_cast<uint4 *>(&lds[buf][b_dst[j]]) = rb[j];
+    };
+
+    floatx4 acc[IB][JB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int r16 = lane & 15, g = lane >> 4;
+    if (s0 < s1) {
+        fetch(s0);
+        stash(0);
+        __syncthreads();
+        for (int s = s0; s < s1; ++s) {
+            const int buf = (s - s0) & 1;
+            const bool more = s + 1 < s1;
+            if (more) fetch(s + 1);
+            const unsigned char *A = &lds[buf][(wm * 64 + r16) * kLdsRow + g * 16];
+            const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint4 a[IB], b[JB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i)
+                    a[i] = *reinterpret_cast<const uint4 *>(A + 16 * i * kLdsRow + h * 64);
+#pragma unroll
+                for (int j = 0; j < JB; ++j)
+                    b[j] = *reinterpret_cast<const uint4 *>(B + 16 * j * kLdsRow + h * 64);
+                Mfma<T>::chunk(acc, a, b);
+            }
+            if (more) stash(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * 64, nb + wn * WTN, lane, tz);
 }
 
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
@@ -480,10 +672,27 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     p.splits = splits;
     p.relu = d.relu ? 1 : 0;
     p.M = (long)batch * d.OD * d.OD * d.OD;
-    const int waves_n = d.N % 256 == 0 ? 4 : d.N % 128 == 0 ? 2 : 1;
-    const int waves_m = 4 / waves_n;
-    dim3 grid((unsigned)((p.M + 64L * waves_m - 1) / (64L * waves_m)), d.N / (64 * waves_n),
-              splits);
+    const bool lds = tdm::conv_uses_lds(sizeof(T) == 4 ? tdm::DType::F32 : tdm::DType::BF16, d);
+    if (lds) {
+        const int bn = d.N % 128 == 0 ? 128 : 64;
+        dim3 grid((unsigned)((p.M + 127) / 128), d.N / bn, splits);
+#define TDM_CONV_LDS(BN)                                                                      \
+    do {                                                                                      \
+        if (splits > 1)                                                                       \
+            conv_lds_kernel<T, OT, BN, false, true><<<grid, 256, 0, s>>>(p);                  \
+        else if (d.pool)                                                                      \
+            conv_lds_kernel<T, OT, BN, true, false><<<grid, 256, 0, s>>>(p);                  \
+        else                                                                                  \
+            conv_lds_kernel<T, OT, BN, false, false><<<grid, 256, 0, s>>>(p);                 \
+    } while (0)
+        if (bn == 128) TDM_CONV_LDS(128);
+        else TDM_CONV_LDS(64);
+#undef TDM_CONV_LDS
+    } else {
+        const int waves_n = d.N % 256 == 0 ? 4 : d.N % 128 == 0 ? 2 : 1;
+        const int waves_m = 4 / waves_n;
+        dim3 grid((unsigned)((p.M + 64L * waves_m - 1) / (64L * waves_m)), d.N / (64 * waves_n),
+                  splits);
 #define TDM_CONV_LAUNCH(WN)                                                                   \
     do {                                                                                      \
         if (splits > 1)                                                                       \
@@ -493,10 +702,11 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         else                                                                                  \
             conv_kernel<T, OT, WN, false, false><<<grid, 256, 0, s>>>(p);                     \
     } while (0)
-    if (waves_n == 4) TDM_CONV_LAUNCH(4);
-    else if (waves_n == 2) TDM_CONV_LAUNCH(2);
-    else TDM_CONV_LAUNCH(1);
+        if (waves_n == 4) TDM_CONV_LAUNCH(4);
+        else if (waves_n == 2) TDM_CONV_LAUNCH(2);
+        else TDM_CONV_LAUNCH(1);
 #undef TDM_CONV_LAUNCH
+    }
     TDM_LAUNCHED();
     if (splits > 1) {
         const long MN = p.M * d.N;
@@ -511,6 +721,24 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
 }  // namespace
 
 namespace tdm {
+
+// The LDS-staged kernel handles channel counts that are whole stages
+// (32 fp32 / 64 bf16 channels); TDM_CONV_IMPL=direct selects the
+// register-direct kernel for every layer (for comparisons).
+bool conv_uses_lds(DType dt, const ConvDesc &d) {
+    static const bool direct = [] {
+        const char *e = std::getenv("TDM_CONV_IMPL");
+        return e && std::strcmp(e, "direct") == 0;
+    }();
+    const int sk = dt == DType::F32 ? 32 : 64;
+    return !direct && d.C % sk == 0 && d.N % 64 == 0;
+}
+
+int conv_reduction_steps(DType dt, const ConvDesc &d) {
+    const int taps = d.ksize * d.ksize * d.ksize;
+    if (conv_uses_lds(dt, d)) return taps * d.C / (dt == DType::F32 ? 32 : 64);
+    return taps * d.C / (dt == DType::F32 ? 16 : 32);
+}
 
 uint16_t float_to_bf16(float f) {
     uint32_t u;

@@ -26,6 +26,11 @@ struct ConvDesc {
     bool pool = false;  // fused 2x2x2 stride-2 max pool (OD even)
 };
 
+// Whether a layer runs on the LDS-staged kernel, and how many reduction
+// steps (the unit split-K divides) its kernel has.
+bool conv_uses_lds(DType dt, const ConvDesc &d);
+int conv_reduction_steps(DType dt, const ConvDesc &d);
+
 // Launches the convolution for `batch` cubes.  out is NDHWC with the output
 // type: float when out_f32, else the storage type `dt`.  `splits` > 1 splits
 // the reduction and needs `partial` (splits*rows*N floats); no pooling then.

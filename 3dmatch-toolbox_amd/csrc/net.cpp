@@ -345,8 +345,7 @@ int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *
         d.ksize = l.k;
         d.relu = l.relu;
         d.pool = l.pool_fused;
-        const size_t kc = net.dt == tdm::DType::F32 ? 16 : 32;
-        const size_t nq = (size_t)l.k * l.k * l.k * l.cin / kc;
+        const size_t nq = (size_t)tdm::conv_reduction_steps(net.dt, d);
         const int splits = choose_splits(net, l, nq);
         void *dst = (l.pool && !l.pool_fused) ? net.act[1 - cur] : out;
         TDM_TRY(tdm::launch_conv(net.dt, out_f32, d, n, net.act[cur], l.w_dev, l.b_dev, dst,
