@@ -496,42 +496,59 @@ struct Conv1Args {
     int relu;
 };
 
-// Value of input voxel (lx, ly, lz) of patch b (local patch coordinates).
-template <bool GRID>
-__device__ inline float fetch1(const Conv1Args &a, long b, int lx, int ly, int lz) {
-    if (GRID) {
-        const int h = a.P / 2;
-        const int x = a.kp[b * 3 + 0] - h + lx, y = a.kp[b * 3 + 1] - h + ly,
-                  z = a.kp[b * 3 + 2] - h + lz;
-        if (x < 0 || y < 0 || z < 0 || x >= a.dx || y >= a.dy || z >= a.dz) return 0.0f;
-        return a.base[((long)z * a.dy + y) * a.dx + x];
-    }
-    return a.base[b * (long)a.P * a.P * a.P + ((long)lz * a.P + ly) * a.P + lx];
-}
-
 template <typename T, typename OT, bool GRID>
 __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
     const int n0 = blockIdx.y * 64;
-    if (m0 >= a.M) return;
+    // No early exit: the epilogue has a workgroup barrier.  Rows past M are
+    // clamped for loads and never stored.
     const int r16 = lane & 15, g = lane >> 4;
     const int OD = a.P - 2;
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int KP = F32 ? 28 : 32;
 
-    long pb[4];
-    int px[4], py[4], pz[4];
+    // Per row block: the input voxel of tap (0,0,0), as a linear index into
+    // the source, its grid coordinates, and whether the whole 3x3x3
+    // neighbourhood is inside the grid (then taps need no bounds test).
+    long vbase[4];
+    int gx[4], gy[4], gz[4];
+    bool inb[4];
+    const int sy = GRID ? a.dx : a.P;                   // source row pitch
+    const long sz = GRID ? (long)a.dx * a.dy : (long)a.P * a.P;  // plane pitch
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         long m = min(m0 + 16 * i + r16, a.M - 1);
-        px[i] = (int)(m % OD);
+        const int px = (int)(m % OD);
         long q = m / OD;
-        py[i] = (int)(q % OD);
+        const int py = (int)(q % OD);
         q /= OD;
-        pz[i] = (int)(q % OD);
-        pb[i] = q / OD;
+        const int pz = (int)(q % OD);
+        const long b = q / OD;
+        if (GRID) {
+            const int h = a.P / 2;
+            gx[i] = a.kp[b * 3 + 0] - h + px;
+            gy[i] = a.kp[b * 3 + 1] - h + py;
+            gz[i] = a.kp[b * 3 + 2] - h + pz;
+            inb[i] = gx[i] >= 0 && gy[i] >= 0 && gz[i] >= 0 && gx[i] + 2 < a.dx &&
+                     gy[i] + 2 < a.dy && gz[i] + 2 < a.dz;
+            vbase[i] = (long)gz[i] * sz + (long)gy[i] * sy + gx[i];
+        } else {
+            gx[i] = px;
+            gy[i] = py;
+            gz[i] = pz;
+            inb[i] = true;
+            vbase[i] = b * (long)a.P * a.P * a.P + (long)pz * sz + (long)py * sy + px;
+        }
     }
+    // Input value of tap t (< 27) for row block i.
+    auto tap_value = [&](int i, int t) -> float {
+        const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+        if (!GRID || inb[i]) return a.base[vbase[i] + kz * sz + ky * sy + kx];
+        const int x = gx[i] + kx, y = gy[i] + ky, z = gz[i] + kz;
+        if (x < 0 || y < 0 || z < 0 || x >= a.dx || y >= a.dy || z >= a.dz) return 0.0f;
+        return a.base[vbase[i] + kz * sz + ky * sy + kx];
+    };
     floatx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -544,11 +561,9 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             const int t = 4 * s + g;
-            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
             float av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                av[i] = t < 27 ? fetch1<GRID>(a, pb[i], px[i] + kx, py[i] + ky, pz[i] + kz) : 0.0f;
+            for (int i = 0; i < 4; ++i) av[i] = t < 27 ? tap_value(i, t) : 0.0f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float *wf = reinterpret_cast<const float *>(w);
@@ -569,9 +584,7 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int t = 8 * g + e;
-                const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
-                float v = t < 27 ? fetch1<GRID>(a, pb[i], px[i] + kx, py[i] + ky, pz[i] + kz) : 0.0f;
-                hv[e] = to_bf16(v);
+                hv[e] = to_bf16(t < 27 ? tap_value(i, t) : 0.0f);
             }
             __builtin_memcpy(&af[i], hv, 16);
         }
@@ -583,20 +596,38 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
         Mfma<bf16_t>::chunk(acc, af, bf);
     }
 
-    OT *__restrict__ out = static_cast<OT *>(a.out);
+    // Epilogue through LDS: the accumulator layout scatters 16-column pieces
+    // of 4 rows per lane; staging the wave's 64x64 tile lets every lane write
+    // whole 16-byte pieces of contiguous output rows (conv1 is bound by its
+    // output writes: 28^3 x 64 values per patch).
+    constexpr int PITCH = 64 + 16 / (int)sizeof(OT);  // padded row, in OT elements
+    __shared__ __attribute__((aligned(16))) OT tile[4][64 * PITCH];
+    OT *__restrict__ t = tile[wave];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const int col = n0 + 16 * j + r16;
-        const float bv = a.bias[col];
+        const int cl = 16 * j + r16;
+        const float bv = a.bias[n0 + cl];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long row = m0 + 16 * i + 4 * g + r;
                 float v = acc[i][j][r] + bv;
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
-                if (row < a.M) store<OT>(out + row * a.N + col, v);
+                store<OT>(t + (16 * i + 4 * g + r) * PITCH + cl, v);
             }
+    }
+    __syncthreads();
+    OT *__restrict__ out = static_cast<OT *>(a.out);
+    constexpr int PIECES = 64 * (int)sizeof(OT) / 16;  // 16-byte pieces per row
+    constexpr int PER = 16 / (int)sizeof(OT);          // elements per piece
+#pragma unroll
+    for (int it = 0; it < 64 * PIECES / 64; ++it) {
+        const int idx = it * 64 + lane;
+        const int rl = idx / PIECES, pc = idx % PIECES;
+        const long row = m0 + rl;
+        if (row < a.M)
+            *reinterpret_cast<uint4 *>(out + row * a.N + n0 + pc * PER) =
+                *reinterpret_cast<const uint4 *>(t + rl * PITCH + pc * PER);
     }
 }
 

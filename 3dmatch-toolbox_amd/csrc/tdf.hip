@@ -176,9 +176,13 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *_
                                                                int dx, int dy, int dz, Lut lut,
                                                                float *__restrict__ tdf) {
     constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
-    __shared__ unsigned sw[HZ * HY * 4];        // halo bit words, x in [X0-32, X0+96)
+    // The halo bit words (x in [X0-32, X0+96)) are dead after pass x, so pass
+    // y's output reuses their LDS (keeps two workgroups per CU at R = 5).
+    constexpr int kSwBytes = HZ * HY * 4 * 4, kS2Bytes = HZ * kTY * kTX;
+    __shared__ __attribute__((aligned(16))) unsigned char shared_a[kSwBytes > kS2Bytes ? kSwBytes : kS2Bytes];
+    unsigned *sw = reinterpret_cast<unsigned *>(shared_a);
+    unsigned char *s2 = shared_a;
     __shared__ unsigned char s1[HZ * HY * kTX];  // pass x, all halo rows
-    __shared__ unsigned char s2[HZ * kTY * kTX]; // pass y
     __shared__ float slut[kLutSize];
 
     const int lane = threadIdx.x & 63;

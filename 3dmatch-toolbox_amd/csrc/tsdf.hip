@@ -20,7 +20,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kZRun = 16;  // z slices swept by one lane
+constexpr int kZRun = 16;  // z slices swept by one lane (one frame)
+constexpr int kZRunF = 8;  // z slices kept in registers across frames
 
 // Per-frame camera constants as the kernel uses them.
 struct Cam {
@@ -55,30 +56,71 @@ __device__ inline ColPart col_part(const Cam &c, float bx, float by) {
     return p;
 }
 
-// The per-voxel body of depth-fusion/demo.cu:34-58 for one frame.
-// Returns true and the truncated signed distance when the voxel is updated.
-__device__ inline bool voxel_sdf(const Cam &c, const ColPart &p, float bz,
-                                 const float *__restrict__ depth, int H, int W, float trunc,
-                                 float &dist) {
+// The per-voxel body of depth-fusion/demo.cu:34-58 for one frame, split into
+// project() (camera coordinates, pixel) and finish() (depth tests, distance).
+//
+// Shortcuts skip work whose outcome is already decided, without changing any
+// result:
+//  - a voxel more than one pixel outside the image is rejected with products
+//    instead of the two divisions (the margin dwarfs the rounding error of
+//    the products; voxels near the border take the exact path);
+//  - with e = d - pcz, the reference's diff = e*s has s = sqrtf(1+qx^2+qy^2)
+//    >= 1, so e <= -trunc implies diff <= -trunc (skip), and e >= trunc
+//    implies diff/trunc >= 1, i.e. dist = 1; the square root and the
+//    division are only evaluated between the two.
+struct Proj {
+    int pix = -1;  // depth-image index, or -1 when the voxel is not seen
+    float pcz = 0.0f, qx = 0.0f, qy = 0.0f;
+};
+
+// depth-fusion/demo.cu:34-49: camera coordinates and pixel of a voxel.
+__device__ inline Proj project(const Cam &c, const ColPart &p, float bz, int H, int W) {
+    Proj r;
     float t2 = bz - c.tz;
     float pcx = p.sx + c.r20 * t2;
     float pcy = p.sy + c.r21 * t2;
     float pcz = p.sz + c.r22 * t2;
-    if (!(pcz > 0.0f)) return false;
-    float qx = pcx / pcz;
-    float qy = pcy / pcz;
+    if (!(pcz > 0.0f)) return r;
+    // fx*pcx/pcz + cx < -1.5, > W + 0.5 (and the same for y): certainly outside.
+    const float lx = c.fx * pcx, ly = c.fy * pcy;
+    if (lx + (c.cx + 1.5f) * pcz < 0.0f || lx + (c.cx - ((float)W + 0.5f)) * pcz > 0.0f ||
+        ly + (c.cy + 1.5f) * pcz < 0.0f || ly + (c.cy - ((float)H + 0.5f)) * pcz > 0.0f)
+        return r;
+    const float qx = pcx / pcz;
+    const float qy = pcy / pcz;
     // roundf = half away from zero; the range test runs on the rounded float,
     // which rejects exactly what the reference's int conversion + test rejects.
-    float rx = roundf(c.fx * qx + c.cx);
-    float ry = roundf(c.fy * qy + c.cy);
-    if (!(rx >= 0.0f && rx < (float)W && ry >= 0.0f && ry < (float)H)) return false;
-    int px = (int)rx, py = (int)ry;
-    float d = depth[py * W + px];
+    const float rx = roundf(c.fx * qx + c.cx);
+    const float ry = roundf(c.fy * qy + c.cy);
+    if (!(rx >= 0.0f && rx < (float)W && ry >= 0.0f && ry < (float)H)) return r;
+    r.pix = (int)ry * W + (int)rx;
+    r.qx = qx;
+    r.qy = qy;
+    r.pcz = pcz;
+    return r;
+}
+
+// depth-fusion/demo.cu:51-62 given the depth sample d of a seen voxel.
+__device__ inline bool finish(const Proj &r, float d, float trunc, float &dist) {
     if (d <= 0.0f || d > 6.0f) return false;
-    float diff = (d - pcz) * sqrtf(1.0f + qx * qx + qy * qy);
+    const float e = d - r.pcz;
+    if (e <= -trunc) return false;
+    if (e >= trunc) {
+        dist = 1.0f;
+        return true;
+    }
+    float diff = e * sqrtf(1.0f + r.qx * r.qx + r.qy * r.qy);
     if (diff <= -trunc) return false;
     dist = fminf(1.0f, diff / trunc);
     return true;
+}
+
+// (t*w + dist)/(w + 1) of depth-fusion/demo.cu:63-66.  When the voxel has
+// only ever been in front of the surface (t == 1) and is again (dist == 1),
+// the quotient is (w + 1)/(w + 1) == 1 exactly, so the division is skipped.
+__device__ inline float running_mean(float t, float w, float w_new, float dist) {
+    if (t == 1.0f && dist == 1.0f) return 1.0f;
+    return (t * w + dist) / w_new;
 }
 
 // One frame; lane = (x, y), blockIdx.y = run of kZRun z slices.
@@ -96,17 +138,22 @@ __global__ __launch_bounds__(kThreads) void integrate_frame_kernel(
     const int z0 = blockIdx.y * kZRun;
     const int z1 = min(dz, z0 + kZRun);
     for (int z = z0; z < z1; ++z) {
+        const Proj r = project(c, p, oz + z * vs, H, W);
+        if (r.pix < 0) continue;
         float dist;
-        if (!voxel_sdf(c, p, oz + z * vs, depth, H, W, trunc, dist)) continue;
+        if (!finish(r, depth[r.pix], trunc, dist)) continue;
         const long i = z * plane + col;
-        float w_old = weight[i];
-        float w_new = w_old + 1.0f;
+        const float w_old = weight[i];
+        const float w_new = w_old + 1.0f;
+        const float t_old = tsdf[i];
+        const float t_new = running_mean(t_old, w_old, w_new, dist);
         weight[i] = w_new;
-        tsdf[i] = (tsdf[i] * w_old + dist) / w_new;
+        // an unchanged value (same bits) needs no store
+        if (__float_as_uint(t_new) != __float_as_uint(t_old)) tsdf[i] = t_new;
     }
 }
 
-// All frames in one pass: a lane keeps the kZRun voxels of its run in
+// All frames in one pass: a lane keeps the kZRunF voxels of its run in
 // registers, applies the frames in order, reads a voxel the first time it is
 // updated and writes back only updated voxels.  Per voxel the sequence of
 // updates is the reference's, so the result is bit-identical.
@@ -122,32 +169,46 @@ __global__ __launch_bounds__(kThreads) void integrate_frames_kernel(
     const int y = (int)(col / dx);
     const float bx = ox + x * vs;
     const float by = oy + y * vs;
-    const int z0 = blockIdx.y * kZRun;
-    const int nz = min(dz - z0, kZRun);
+    const int z0 = blockIdx.y * kZRunF;
+    const int nz = min(dz - z0, kZRunF);
     const long base = z0 * plane + col;
-    float t[kZRun], w[kZRun];
+    float t[kZRunF], w[kZRunF];
     unsigned loaded = 0;
     const long frame_px = (long)H * W;
     for (int f = 0; f < F; ++f) {
         const Cam c = load_cam(K, cam2base + f * 16);
         const ColPart p = col_part(c, bx, by);
         const float *__restrict__ depth = depths + f * frame_px;
+        Proj pr[kZRunF];
 #pragma unroll
-        for (int j = 0; j < kZRun; ++j) {
-            float dist;
-            if (j >= nz || !voxel_sdf(c, p, oz + (z0 + j) * vs, depth, H, W, trunc, dist)) continue;
-            if (!(loaded & (1u << j))) {
+        for (int j = 0; j < kZRunF; ++j)
+            pr[j] = j < nz ? project(c, p, oz + (z0 + j) * vs, H, W) : Proj{};
+        float d[kZRunF];
+#pragma unroll
+        for (int j = 0; j < kZRunF; ++j) d[j] = depth[max(pr[j].pix, 0)];
+        unsigned upd = 0;
+        float dist[kZRunF];
+#pragma unroll
+        for (int j = 0; j < kZRunF; ++j)
+            if (pr[j].pix >= 0 && finish(pr[j], d[j], trunc, dist[j])) upd |= 1u << j;
+        const unsigned fresh = upd & ~loaded;
+#pragma unroll
+        for (int j = 0; j < kZRunF; ++j)
+            if (fresh & (1u << j)) {
                 t[j] = tsdf[base + j * plane];
                 w[j] = weight[base + j * plane];
-                loaded |= 1u << j;
             }
-            float w_new = w[j] + 1.0f;
-            t[j] = (t[j] * w[j] + dist) / w_new;
-            w[j] = w_new;
-        }
+        loaded |= upd;
+#pragma unroll
+        for (int j = 0; j < kZRunF; ++j)
+            if (upd & (1u << j)) {
+                const float w_new = w[j] + 1.0f;
+                t[j] = running_mean(t[j], w[j], w_new, dist[j]);
+                w[j] = w_new;
+            }
     }
 #pragma unroll
-    for (int j = 0; j < kZRun; ++j) {
+    for (int j = 0; j < kZRunF; ++j) {
         if (loaded & (1u << j)) {
             tsdf[base + j * plane] = t[j];
             weight[base + j * plane] = w[j];
@@ -229,7 +290,7 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     if (num_frames == 0) return TDM_OK;
     long plane = (long)dim_x * dim_y;
-    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRun - 1) / kZRun);
+    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRunF - 1) / kZRunF);
     TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
     integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
         cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z,

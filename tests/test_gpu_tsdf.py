@@ -67,6 +67,22 @@ def test_synthetic_frames_bit_exact(torch_gpu, synth_frames, fused):
     _assert_same(got, ref)
 
 
+def test_full_size_grid_bit_exact(torch_gpu):
+    """The benchmark's 256^3 grid at 1 cm (first 12 of its 50 frames)."""
+    from tdm import synth
+
+    mm, c2w = synth.depth_frames(50, count=12)
+    depths = synth.depth_to_metres(mm)
+    poses = synth.relative_poses(c2w)
+    K = synth.intrinsics()
+    args = (synth.TSDF_DIMS, synth.TSDF_ORIGIN, synth.TSDF_VOXEL, synth.TSDF_TRUNC)
+    print("case 256^3 x 12 frames", flush=True)
+    ref = _oracle_fuse(K, poses, depths, *args)
+    assert ref[1].sum() > 1e7
+    for fused in (False, True):
+        _assert_same(_gpu_fuse(torch_gpu, K, poses, depths, *args, fused), ref)
+
+
 def test_sample_frames_bit_exact(torch_gpu):
     """The reference's own sample RGB-D frames (data/sample/depth-fusion-demo),
     5 of them committed as fixtures, fused with the demo's pose arithmetic."""
