@@ -35,9 +35,15 @@ namespace {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t bf16_t;
+typedef _Float16 f16_t;
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 template <typename T>
 struct Kc;
+template <>
+struct Kc<f16_t> {
+    static constexpr int value = 32;
+};
 template <>
 struct Kc<float> {
     static constexpr int value = 16;  // k per 16-byte lane load x 4 k-groups
@@ -92,6 +98,23 @@ struct Mfma<bf16_t> {
     }
 };
 
+template <>
+struct Mfma<f16_t> {
+    template <int IB, int JB>
+    __device__ static inline void chunk(floatx4 (&acc)[IB][JB], const uint4 (&a)[IB],
+                                        const uint4 (&b)[JB]) {
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+#pragma unroll
+            for (int j = 0; j < JB; ++j) {
+                halfx8 av, bv;
+                __builtin_memcpy(&av, &a[i], 16);
+                __builtin_memcpy(&bv, &b[j], 16);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[i][j], 0, 0, 0);
+            }
+    }
+};
+
 __device__ inline bf16_t to_bf16(float f) {
     unsigned u = __float_as_uint(f);
     if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
@@ -110,6 +133,31 @@ __device__ inline void store<float>(float *p, float v) {
 template <>
 __device__ inline void store<bf16_t>(bf16_t *p, float v) {
     *p = to_bf16(v);
+}
+template <>
+__device__ inline void store<f16_t>(f16_t *p, float v) {
+    *p = (f16_t)v;  // round to nearest even, overflow to inf
+}
+
+// 16-bit storage bits of a float for the MFMA operand types.
+template <typename T>
+__device__ inline unsigned short bits16(float v);
+template <>
+__device__ inline unsigned short bits16<bf16_t>(float v) {
+    return to_bf16(v);
+}
+template <>
+__device__ inline unsigned short bits16<f16_t>(float v) {
+    f16_t h = (f16_t)v;
+    unsigned short u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+
+template <typename T>
+constexpr tdm::DType dtype_of() {
+    return sizeof(T) == 4 ? tdm::DType::F32
+                          : (__is_same(T, f16_t) ? tdm::DType::F16 : tdm::DType::BF16);
 }
 
 struct ConvArgs {
@@ -556,7 +604,7 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
         for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const T *__restrict__ w = static_cast<const T *>(a.w);
-    if (F32) {
+    if constexpr (F32) {
         // 7 k-steps of 4 taps: lane group g supplies tap 4s + g.
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
@@ -584,16 +632,13 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int t = 8 * g + e;
-                hv[e] = to_bf16(t < 27 ? tap_value(i, t) : 0.0f);
+                hv[e] = bits16<T>(t < 27 ? tap_value(i, t) : 0.0f);
             }
             __builtin_memcpy(&af[i], hv, 16);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const bf16_t *wb = reinterpret_cast<const bf16_t *>(w);
-            bf[j] = ld16(wb + (long)(n0 + 16 * j + r16) * KP + 8 * g);
-        }
-        Mfma<bf16_t>::chunk(acc, af, bf);
+        for (int j = 0; j < 4; ++j) bf[j] = ld16(w + (long)(n0 + 16 * j + r16) * KP + 8 * g);
+        Mfma<T>::chunk(acc, af, bf);
     }
 
     // Epilogue through LDS: the accumulator layout scatters 16-column pieces
@@ -642,6 +687,10 @@ __device__ inline float load_as_float<float>(const float *p) {
 template <>
 __device__ inline float load_as_float<bf16_t>(const bf16_t *p) {
     return from_bf16(*p);
+}
+template <>
+__device__ inline float load_as_float<f16_t>(const f16_t *p) {
+    return (float)*p;
 }
 
 template <typename T>
@@ -703,7 +752,7 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     p.splits = splits;
     p.relu = d.relu ? 1 : 0;
     p.M = (long)batch * d.OD * d.OD * d.OD;
-    const bool lds = tdm::conv_uses_lds(sizeof(T) == 4 ? tdm::DType::F32 : tdm::DType::BF16, d);
+    const bool lds = tdm::conv_uses_lds(dtype_of<T>(), d);
     if (lds) {
         const int bn = d.N % 128 == 0 ? 128 : 64;
         dim3 grid((unsigned)((p.M + 127) / 128), d.N / bn, splits);
@@ -771,6 +820,13 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
     return taps * d.C / (dt == DType::F32 ? 16 : 32);
 }
 
+uint16_t float_to_f16(float f) {
+    const _Float16 h = (_Float16)f;  // IEEE conversion, round to nearest even
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
 uint16_t float_to_bf16(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -791,6 +847,10 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
               "bad split");
     if (batch <= 0) return TDM_OK;
     if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, s);
+    if (dt == DType::F16) {
+        if (out_f32) return conv_dispatch<f16_t, float>(d, batch, in, w, bias, out, splits, partial, s);
+        return conv_dispatch<f16_t, f16_t>(d, batch, in, w, bias, out, splits, partial, s);
+    }
     if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, s);
     return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, s);
 }
@@ -822,6 +882,8 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
         else conv1_kernel<T, OT, false><<<grid, 256, 0, s>>>(a);                              \
     } while (0)
     if (dt == DType::F32) TDM_C1(float, float);
+    else if (dt == DType::F16 && out_f32) TDM_C1(f16_t, float);
+    else if (dt == DType::F16) TDM_C1(f16_t, f16_t);
     else if (out_f32) TDM_C1(bf16_t, float);
     else TDM_C1(bf16_t, bf16_t);
 #undef TDM_C1
@@ -836,6 +898,8 @@ int launch_to_ncdhw(DType dt, bool in_f32, const void *in, int batch, int C, int
     unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
     if (dt == DType::F32 || in_f32)
         to_ncdhw_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E, out);
+    else if (dt == DType::F16)
+        to_ncdhw_kernel<f16_t><<<blocks, 256, 0, s>>>(static_cast<const f16_t *>(in), batch, C, E, out);
     else
         to_ncdhw_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C, E, out);
     TDM_LAUNCHED();
@@ -850,6 +914,9 @@ int launch_maxpool(DType dt, const void *in, int batch, int C, int E, void *out,
     if (dt == DType::F32)
         maxpool_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E,
                                                      OE, static_cast<float *>(out));
+    else if (dt == DType::F16)
+        maxpool_kernel<f16_t><<<blocks, 256, 0, s>>>(static_cast<const f16_t *>(in), batch, C, E,
+                                                     OE, static_cast<f16_t *>(out));
     else
         maxpool_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C,
                                                       E, OE, static_cast<bf16_t *>(out));

@@ -9,7 +9,7 @@
 
 namespace tdm {
 
-enum class DType { F32 = 0, BF16 = 1 };
+enum class DType { F32 = 0, BF16 = 1, F16 = 2 };
 
 inline size_t dtype_size(DType t) { return t == DType::F32 ? 4 : 2; }
 
@@ -63,7 +63,10 @@ int launch_to_ncdhw(DType dt, bool in_f32, const void *in, int batch, int C, int
 // out extent 1 + ceil((E - 2)/2), windows clipped.
 int launch_maxpool(DType dt, const void *in, int batch, int C, int E, void *out, hipStream_t s);
 
-// Host float -> bf16 bits, round to nearest even.
+// Host float -> bf16 / IEEE half bits, round to nearest even (half: overflow
+// to infinity, gradual underflow; the rounding of cpu_float2half,
+// core/marvin.hpp:467-521).
 uint16_t float_to_bf16(float f);
+uint16_t float_to_f16(float f);
 
 }  // namespace tdm

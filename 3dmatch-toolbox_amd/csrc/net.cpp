@@ -243,7 +243,10 @@ int upload_layer(tdm_net &net, Layer &l, size_t li) {
         TDM_HIP(hipMemcpy(l.w_dev, packed.data(), bytes, hipMemcpyHostToDevice));
     } else {
         std::vector<uint16_t> h(packed.size());
-        for (size_t i = 0; i < h.size(); ++i) h[i] = tdm::float_to_bf16(packed[i]);
+        if (net.dt == tdm::DType::F16)
+            for (size_t i = 0; i < h.size(); ++i) h[i] = tdm::float_to_f16(packed[i]);
+        else
+            for (size_t i = 0; i < h.size(); ++i) h[i] = tdm::float_to_bf16(packed[i]);
         TDM_HIP(hipMemcpy(l.w_dev, h.data(), bytes, hipMemcpyHostToDevice));
     }
     TDM_HIP(hipMemcpy(l.b_dev, l.b.data(), sizeof(float) * l.cout, hipMemcpyHostToDevice));
@@ -431,8 +434,9 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
                    tdm_net **out) {
     tdm::clear_error();
     TDM_CHECK(json_path && out, TDM_ERR_INVALID_ARG, "null pointer argument");
-    TDM_CHECK(precision == TDM_PREC_FP32 || precision == TDM_PREC_BF16, TDM_ERR_INVALID_ARG,
-              "unknown precision %d", precision);
+    TDM_CHECK(precision == TDM_PREC_FP32 || precision == TDM_PREC_BF16 ||
+                  precision == TDM_PREC_FP16,
+              TDM_ERR_INVALID_ARG, "unknown precision %d", precision);
     TDM_CHECK(max_batch >= 0 && max_batch <= 65536, TDM_ERR_INVALID_ARG, "bad max_batch");
     std::string text, err;
     TDM_TRY(read_text(json_path, text));
@@ -441,7 +445,9 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
     tdm_net *net = new tdm_net();
     net->ctx = ctx;
     net->device = ctx ? ctx->device : -1;
-    net->dt = precision == TDM_PREC_FP32 ? tdm::DType::F32 : tdm::DType::BF16;
+    net->dt = precision == TDM_PREC_FP32   ? tdm::DType::F32
+              : precision == TDM_PREC_FP16 ? tdm::DType::F16
+                                           : tdm::DType::BF16;
     net->max_batch = max_batch ? max_batch : 256;
     int s = build_layers(root, *net);
     if (s != TDM_OK) {

@@ -31,6 +31,7 @@ ERR_NO_DEVICE = 7
 
 PREC_FP32 = 0
 PREC_BF16 = 1
+PREC_FP16 = 2
 DECODE_EXACT = 0
 DECODE_REFERENCE = 1
 

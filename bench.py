@@ -151,6 +151,92 @@ def tsdf_bench(torch, reps=3):
     return out
 
 
+def mutual_nn(torch, a, b):
+    """Nearest neighbours of a's rows in b and mutual pairs, by L2 distance
+    (the matching of core/demo.m:49-55)."""
+    d = torch.cdist(a, b)
+    ab = d.argmin(dim=1)
+    ba = d.argmin(dim=0)
+    idx = torch.arange(len(a), device=a.device)
+    mutual = ba[ab] == idx
+    return ab, mutual
+
+
+def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8):
+    """Precision form: fragments = views 0..n_frag-1 of the synthetic scene,
+    n_kp keypoints each; bf16 descriptors vs the fp32 path (rel-L2), mutual-NN
+    agreement on the pair (0, 1), and per-layer rel-L2 of both paths vs the
+    FP64 oracle on a few patches."""
+    import numpy as np
+
+    import tdm
+    from tdm import netdef, synth
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    dev = torch.device("cuda:0")
+    nets = {p: tdm.Net(js, p, max_batch=1024)
+            for p in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16)}
+    for n in nets.values():
+        n.load_weights(wt)
+    desc = {p: [] for p in nets}
+    first_patches = None
+    for f in range(n_frag):
+        cloud = synth.fragment(200_000, view=f, seed=10 + f)
+        pts = torch.from_numpy(cloud).to(dev)
+        dims, origin = tdm.grid_params(pts, 0.01, 15)
+        tdf = torch.empty(dims[2], dims[1], dims[0], device=dev)
+        tdm.tdf_from_points(pts, dims, origin, np.float32(0.01), np.float32(0.05), tdf)
+        kp = cloud[synth.keypoint_indices(len(cloud), n_kp, seed=100 + f)]
+        kg = tdm.keypoints_to_grid(kp, origin, np.float32(0.01))
+        kgd = torch.from_numpy(kg).to(dev)
+        for p, net in nets.items():
+            out = torch.empty(n_kp, 512, device=dev)
+            net.forward_grid(tdf, dims, kgd, out)
+            desc[p].append(out)
+        if f == 0:
+            patches = torch.empty(layer_patches, 30, 30, 30, device=dev)
+            tdm.extract_patches(tdf, dims, kgd[:layer_patches], patches)
+            first_patches = patches
+    torch.cuda.synchronize()
+    d32 = torch.cat(desc[tdm.PREC_FP32])
+    ab32, m32 = mutual_nn(torch, desc[tdm.PREC_FP32][0], desc[tdm.PREC_FP32][1])
+    vs_fp32 = {}
+    for p, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+        d16 = torch.cat(desc[p])
+        ab16, m16 = mutual_nn(torch, desc[p][0], desc[p][1])
+        both = m32 & m16 & (ab32 == ab16)
+        vs_fp32[name] = {
+            "descriptor_rel_l2": float(torch.linalg.norm(d16 - d32) / torch.linalg.norm(d32)),
+            "nn_agreement_pair01": float((ab32 == ab16).float().mean()),
+            "mutual_nn_agreement_pair01": float(both.sum()) / max(int(m32.sum()), 1),
+        }
+    # per-layer errors vs FP64 on the first fragment's first patches
+    shapes = netdef.layer_shapes()
+    p_host = first_patches.cpu().numpy()
+    ref = oracle.net_forward_fp64(p_host, dict(weights), shapes)
+    per_layer = {}
+    for p, name in ((tdm.PREC_FP32, "fp32"), (tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+        errs = []
+        for i, (_n, _ci, co, _k, _d, od, pooled) in enumerate(shapes):
+            e = pooled if pooled is not None else od
+            out = torch.empty(layer_patches, co, e, e, e, device=dev)
+            nets[p].forward_layer(first_patches, i, out)
+            got = out.cpu().numpy().astype(np.float64)
+            errs.append(float(np.linalg.norm(got - ref[i]) / np.linalg.norm(ref[i])))
+        per_layer[name] = dict(zip([s[0] for s in shapes], errs))
+    for n in nets.values():
+        n.close()
+    return {
+        "fragments": n_frag, "keypoints_per_fragment": n_kp,
+        "vs_fp32": vs_fp32,
+        "mutual_nn_pairs_fp32_pair01": int(m32.sum()),
+        "per_layer_rel_l2_vs_fp64": per_layer,
+        "per_layer_patches": layer_patches,
+    }
+
+
 def cpu_baseline(cloud, keypts, weights, n_kp, conv_patches=32):
     """Oracle (C loop nests, OpenMP) on a bounded sample of the descriptor
     workload: TDF of 1/8 of the grid's voxels and `conv_patches` patches
@@ -295,20 +381,25 @@ def main() -> int:
     torch.cuda.empty_cache()
 
     if not args.no_extras:
-        log("bf16 descriptor pipeline")
-        pb = DescriptorPipeline(torch, cloud, keypts, js, wt, tdm.PREC_BF16, args.batch)
-        pb.net.set_profiling(True)
-        sb = time_steps(torch, pb.step, args.steps, args.warmup)
-        ms2b, passes2b, patches2b = pb.net.layer_stats(1)
-        result["bf16"] = {
-            "descriptors_per_s": n_kp / sb,
-            "ms_per_step": sb * 1e3,
-            "speedup_vs_fp32": (n_kp / sb) / value,
-            "conv2_tflops": conv2_flops * patches2b / (ms2b / 1e3) / 1e12,
-            "conv2_frac_of_bf16_peak": conv2_flops * patches2b / (ms2b / 1e3) / 1e12 / MFMA_PEAK_TFLOPS["bf16"],
-        }
-        pb.net.close()
-        del pb
+        for prec, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+            log(f"{name} descriptor pipeline")
+            pb = DescriptorPipeline(torch, cloud, keypts, js, wt, prec, args.batch)
+            pb.net.set_profiling(True)
+            sb = time_steps(torch, pb.step, args.steps, args.warmup)
+            ms2b, passes2b, patches2b = pb.net.layer_stats(1)
+            tf = conv2_flops * patches2b / (ms2b / 1e3) / 1e12
+            result[name] = {
+                "descriptors_per_s": n_kp / sb,
+                "ms_per_step": sb * 1e3,
+                "speedup_vs_fp32": (n_kp / sb) / value,
+                "conv2_tflops": tf,
+                "conv2_frac_of_16bit_peak": tf / MFMA_PEAK_TFLOPS["bf16"],
+            }
+            pb.net.close()
+            del pb
+            torch.cuda.empty_cache()
+        log("precision study")
+        result["precision"] = precision_study(torch, js, wt, weights)
         torch.cuda.empty_cache()
         log("tsdf fusion")
         result["tsdf"] = tsdf_bench(torch)

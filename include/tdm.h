@@ -47,6 +47,9 @@ enum tdm_status {
 enum tdm_precision {
     TDM_PREC_FP32 = 0, /* float storage, float MFMA (reference DATATYPE 1)      */
     TDM_PREC_BF16 = 1, /* bf16 weights/activations, fp32 accumulation on MFMA   */
+    TDM_PREC_FP16 = 2, /* fp16 weights/activations, fp32 accumulation: the half
+                          storage / float compute of the reference's default
+                          build (DATATYPE 0, core/marvin.hpp:11-27)          */
 };
 
 /* How ComputeTDF turns a linear voxel index into (x, y, z). */

@@ -3,7 +3,8 @@ against the FP64 oracle (torch CPU float64 conv3d/relu/max_pool3d).
 
 Tolerances (relative L2 over each layer's whole output, NCDHW):
   fp32 path  <= 1e-5 per layer and for the descriptors;
-  bf16 path  <= 3e-2 per layer, <= 3e-2 for the descriptors.
+  bf16 path  <= 3e-2 per layer, <= 3e-2 for the descriptors;
+  fp16 path  <= 1e-2 per layer and for the descriptors.
 """
 import os
 
@@ -19,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 FP32_TOL = 1e-5
 BF16_TOL = 3e-2
+FP16_TOL = 1e-2
 
 
 @pytest.fixture(scope="module")
@@ -67,7 +69,10 @@ def _run_layers(torch, net, p):
     return outs
 
 
-@pytest.mark.parametrize("prec,tol", [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL)])
+PRECISIONS = [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL), (tdm.PREC_FP16, FP16_TOL)]
+
+
+@pytest.mark.parametrize("prec,tol", PRECISIONS)
 def test_per_layer_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
     js, wt, _ = net_files
     net = tdm.Net(js, prec, max_batch=4)
@@ -79,7 +84,7 @@ def test_per_layer_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
         assert e <= tol, f"{name}: rel-L2 {e:.3e} > {tol}"
 
 
-@pytest.mark.parametrize("prec,tol", [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL)])
+@pytest.mark.parametrize("prec,tol", PRECISIONS)
 def test_descriptors_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
     js, wt, _ = net_files
     net = tdm.Net(js, prec)
@@ -98,7 +103,7 @@ def test_grid_path_equals_patch_path(torch_gpu, net_files, patches):
     """Patch gather fused into conv1 gives bit-identical descriptors."""
     js, wt, _ = net_files
     p, tdf, dims, kg = patches
-    for prec in (tdm.PREC_FP32, tdm.PREC_BF16):
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
         net = tdm.Net(js, prec)
         net.load_weights(wt)
         a = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
