@@ -534,6 +534,8 @@ __global__ void split_reduce_kernel(const float *__restrict__ part, int splits, 
 struct Conv1Args {
     const float *base;
     const int *kp;
+    const tdm::GridRef *grids;  // multi-grid mode: table of grids, indexed by gid
+    const int *gid;
     int dx, dy, dz;
     int P;  // patch extent; output extent P - 2
     const void *w;
@@ -544,8 +546,11 @@ struct Conv1Args {
     int relu;
 };
 
-template <typename T, typename OT, bool GRID>
+// MODE 0: patches [batch][P^3]; 1: keypoints in one grid; 2: keypoints in
+// a table of grids (gid[b] picks patch b's grid).
+template <typename T, typename OT, int MODE>
 __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
+    constexpr bool GRID = MODE != 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
     const int n0 = blockIdx.y * 64;
@@ -556,14 +561,14 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int KP = F32 ? 28 : 32;
 
-    // Per row block: the input voxel of tap (0,0,0), as a linear index into
-    // the source, its grid coordinates, and whether the whole 3x3x3
-    // neighbourhood is inside the grid (then taps need no bounds test).
-    long vbase[4];
+    // Per row block: the source voxel of tap (0,0,0) (pointer), its grid
+    // coordinates, the grid's extents and pitches, and whether the whole
+    // 3x3x3 neighbourhood is inside the grid (then taps need no bounds test).
+    const float *vb[4];
     int gx[4], gy[4], gz[4];
     bool inb[4];
-    const int sy = GRID ? a.dx : a.P;                   // source row pitch
-    const long sz = GRID ? (long)a.dx * a.dy : (long)a.P * a.P;  // plane pitch
+    int ex[4], ey[4], ez[4], sy[4];
+    long sz[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         long m = min(m0 + 16 * i + r16, a.M - 1);
@@ -574,28 +579,45 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
         const int pz = (int)(q % OD);
         const long b = q / OD;
         if (GRID) {
+            const float *base = a.base;
+            ex[i] = a.dx;
+            ey[i] = a.dy;
+            ez[i] = a.dz;
+            if (MODE == 2) {
+                const tdm::GridRef &gr = a.grids[a.gid[b]];
+                base = gr.tdf;
+                ex[i] = gr.dims[0];
+                ey[i] = gr.dims[1];
+                ez[i] = gr.dims[2];
+            }
+            sy[i] = ex[i];
+            sz[i] = (long)ex[i] * ey[i];
             const int h = a.P / 2;
             gx[i] = a.kp[b * 3 + 0] - h + px;
             gy[i] = a.kp[b * 3 + 1] - h + py;
             gz[i] = a.kp[b * 3 + 2] - h + pz;
-            inb[i] = gx[i] >= 0 && gy[i] >= 0 && gz[i] >= 0 && gx[i] + 2 < a.dx &&
-                     gy[i] + 2 < a.dy && gz[i] + 2 < a.dz;
-            vbase[i] = (long)gz[i] * sz + (long)gy[i] * sy + gx[i];
+            inb[i] = gx[i] >= 0 && gy[i] >= 0 && gz[i] >= 0 && gx[i] + 2 < ex[i] &&
+                     gy[i] + 2 < ey[i] && gz[i] + 2 < ez[i];
+            vb[i] = base + ((long)gz[i] * sz[i] + (long)gy[i] * sy[i] + gx[i]);
         } else {
             gx[i] = px;
             gy[i] = py;
             gz[i] = pz;
             inb[i] = true;
-            vbase[i] = b * (long)a.P * a.P * a.P + (long)pz * sz + (long)py * sy + px;
+            ex[i] = ey[i] = ez[i] = a.P;
+            sy[i] = a.P;
+            sz[i] = (long)a.P * a.P;
+            vb[i] = a.base + (b * (long)a.P * a.P * a.P + (long)pz * sz[i] + (long)py * sy[i] + px);
         }
     }
-    // Input value of tap t (< 27) for row block i.
+    // Input value of tap t (< 27) for row block i.  Outside the grid reads 0;
+    // the address is formed only for taps inside it.
     auto tap_value = [&](int i, int t) -> float {
         const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
-        if (!GRID || inb[i]) return a.base[vbase[i] + kz * sz + ky * sy + kx];
+        if (!GRID || inb[i]) return vb[i][kz * sz[i] + ky * sy[i] + kx];
         const int x = gx[i] + kx, y = gy[i] + ky, z = gz[i] + kz;
-        if (x < 0 || y < 0 || z < 0 || x >= a.dx || y >= a.dy || z >= a.dz) return 0.0f;
-        return a.base[vbase[i] + kz * sz + ky * sy + kx];
+        if (x < 0 || y < 0 || z < 0 || x >= ex[i] || y >= ey[i] || z >= ez[i]) return 0.0f;
+        return vb[i][kz * sz[i] + ky * sy[i] + kx];
     };
     floatx4 acc[4][4];
 #pragma unroll
@@ -875,11 +897,14 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
     const long OD = src.P - 2;
     a.M = (long)batch * OD * OD * OD;
     dim3 grid((unsigned)((a.M + 255) / 256), N / 64);
-    const bool gridmode = src.kp != nullptr;
+    a.grids = src.grids;
+    a.gid = src.gid;
+    const int mode = src.kp == nullptr ? 0 : src.grids == nullptr ? 1 : 2;
 #define TDM_C1(T, OT)                                                                         \
     do {                                                                                      \
-        if (gridmode) conv1_kernel<T, OT, true><<<grid, 256, 0, s>>>(a);                      \
-        else conv1_kernel<T, OT, false><<<grid, 256, 0, s>>>(a);                              \
+        if (mode == 2) conv1_kernel<T, OT, 2><<<grid, 256, 0, s>>>(a);                        \
+        else if (mode == 1) conv1_kernel<T, OT, 1><<<grid, 256, 0, s>>>(a);                   \
+        else conv1_kernel<T, OT, 0><<<grid, 256, 0, s>>>(a);                                  \
     } while (0)
     if (dt == DType::F32) TDM_C1(float, float);
     else if (dt == DType::F16 && out_f32) TDM_C1(f16_t, float);

@@ -38,12 +38,22 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
                 const void *w, const float *bias, void *out, int splits, float *partial,
                 hipStream_t s);
 
+// One TDF grid of a multi-grid pass (device table entry; the layout of
+// tdm_grid_ref).
+struct GridRef {
+    const float *tdf;
+    int dims[3];
+    int reserved;
+};
+
 // Where the first (single-channel) convolution reads its input.
 struct VolumeSource {
-    const float *base = nullptr;  // patches [batch][P^3] or a TDF grid
-    const int *kp = nullptr;      // grid mode: keypoint grid coords [batch][3]
-    int dims[3] = {0, 0, 0};      // grid mode: grid dims
-    int P = 30;                   // patch extent
+    const float *base = nullptr;     // patches [batch][P^3] or a TDF grid
+    const int *kp = nullptr;         // grid modes: keypoint grid coords [batch][3]
+    int dims[3] = {0, 0, 0};         // single-grid mode: grid dims
+    const GridRef *grids = nullptr;  // multi-grid mode: device table of grids
+    const int *gid = nullptr;        // multi-grid mode: grid of each patch [batch]
+    int P = 30;                      // patch extent
 };
 
 // First convolution: 1 input channel, N output channels (multiple of 64),

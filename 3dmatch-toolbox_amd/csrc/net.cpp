@@ -10,6 +10,7 @@
 // storage type of the precision (float or bf16); the last layer writes float
 // descriptors straight into the caller's buffer.
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -608,6 +609,30 @@ int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, i
         src.dims[0] = dim_x;
         src.dims[1] = dim_y;
         src.dims[2] = dim_z;
+        src.P = net->patch;
+        TDM_TRY(run_pass(*net, src, n, (int)net->layers.size() - 1, feat + (long)start * D, nullptr, s));
+    }
+    return TDM_OK;
+}
+
+int tdm_net_forward_grids(tdm_net *net, const tdm_grid_ref *grids, const int *kp_grid,
+                          const int *grid_index, int num, float *feat, void *stream) {
+    static_assert(sizeof(tdm_grid_ref) == sizeof(tdm::GridRef) &&
+                      offsetof(tdm_grid_ref, dim_x) == offsetof(tdm::GridRef, dims),
+                  "tdm_grid_ref and GridRef layouts differ");
+    tdm::clear_error();
+    TDM_CHECK(num >= 0, TDM_ERR_INVALID_ARG, "negative count");
+    TDM_CHECK(num == 0 || (grids && kp_grid && grid_index && feat), TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_TRY(check_ready(net));
+    hipStream_t s = tdm::as_stream(stream);
+    const int D = net->layers.back().cout;
+    for (int start = 0; start < num; start += net->max_batch) {
+        const int n = std::min(net->max_batch, num - start);
+        tdm::VolumeSource src;
+        src.kp = kp_grid + (long)start * 3;
+        src.grids = reinterpret_cast<const tdm::GridRef *>(grids);
+        src.gid = grid_index + start;
         src.P = net->patch;
         TDM_TRY(run_pass(*net, src, n, (int)net->layers.size() - 1, feat + (long)start * D, nullptr, s));
     }

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pipeline.hpp"
 
 #pragma clang fp contract(off)
 
@@ -396,6 +397,27 @@ int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, unsigned **bits, size_t *
 
 }  // namespace
 
+namespace tdm {
+
+int grid_from_box(const float box[6], float voxel_size, int padding, int dims[3],
+                  float origin[3]) {
+    // core/demo.cu:116-122, float arithmetic on the host.
+    for (int a = 0; a < 3; ++a) {
+        TDM_CHECK(std::isfinite(box[a]) && std::isfinite(box[3 + a]), TDM_ERR_INVALID_ARG,
+                  "non-finite point coordinates");
+        volatile float ext = (box[3 + a] - box[a]) / voxel_size;
+        double dim = std::round((float)ext) + 1.0 + padding * 2.0;
+        TDM_CHECK(dim < 2147483647.0, TDM_ERR_INVALID_ARG, "grid extent too large");
+        dims[a] = (int)dim;
+        volatile float padw = padding * voxel_size;
+        volatile float o = box[a] - padw;
+        origin[a] = o + voxel_size / 2;
+    }
+    return TDM_OK;
+}
+
+}  // namespace tdm
+
 extern "C" {
 
 int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_size, int padding,
@@ -418,19 +440,7 @@ int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_siz
     float box[6];
     TDM_HIP(hipMemcpyAsync(box, res, sizeof box, hipMemcpyDeviceToHost, s));
     TDM_HIP(hipStreamSynchronize(s));
-    // core/demo.cu:116-122, float arithmetic on the host.
-    for (int a = 0; a < 3; ++a) {
-        TDM_CHECK(std::isfinite(box[a]) && std::isfinite(box[3 + a]), TDM_ERR_INVALID_ARG,
-                  "non-finite point coordinates");
-        volatile float ext = (box[3 + a] - box[a]) / voxel_size;
-        double dim = std::round((float)ext) + 1.0 + padding * 2.0;
-        TDM_CHECK(dim < 2147483647.0, TDM_ERR_INVALID_ARG, "grid extent too large");
-        dims_out[a] = (int)dim;
-        volatile float padw = padding * voxel_size;
-        volatile float o = box[a] - padw;
-        origin_out[a] = o + voxel_size / 2;
-    }
-    return TDM_OK;
+    return tdm::grid_from_box(box, voxel_size, padding, dims_out, origin_out);
 }
 
 int tdm_occupancy(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],

@@ -173,6 +173,138 @@ def extract_patches(tdf, dims, kp_grid, out, patch_size=30, stream=None) -> None
                                      _p(out), _stream(stream)))
 
 
+# ---- registration -----------------------------------------------------------------
+
+
+def match_nn(desc_a, desc_b, nn_ab=None, nn_ba=None, dist_ab=None, dist_ba=None, refined=None,
+             stream=None) -> None:
+    """Nearest neighbours both ways (knnsearch, core/demo.m:49-52) into the
+    given device int32/float32 tensors; refined: device int32[2] or None."""
+    na, dim = desc_a.shape
+    nb = desc_b.shape[0]
+    assert desc_b.shape[1] == dim
+    _check(lib().tdm_match_nn(None, _p(desc_a), c_int(na), _p(desc_b), c_int(nb), c_int(dim),
+                              _p(nn_ab), _p(dist_ab), _p(nn_ba), _p(dist_ba), _p(refined),
+                              _stream(stream)))
+
+
+def mutual_pairs(nn_ab, nn_ba, pairs, num_pairs, stream=None) -> None:
+    """Mutual pairs (core/demo.m:53-55): pairs int32 [nb, 2], num_pairs int32 [1]."""
+    _check(lib().tdm_mutual_pairs(None, _p(nn_ab), c_int(nn_ab.shape[0]), _p(nn_ba),
+                                  c_int(nn_ba.shape[0]), _p(pairs), _p(num_pairs),
+                                  _stream(stream)))
+
+
+def ransac_rt(kp1, kp2, rt, inliers, stats, pairs=None, num_pairs=None, max_pairs=None,
+              threshold=0.05, max_trials=50000, seed=0, trial_counts=None, stream=None) -> None:
+    """ransacfitRt (core/external/sfm/ransacfitRt.m) on the device.
+    rt: float64 [12]; inliers: int32 [max_pairs]; stats: int32 [4]."""
+    if max_pairs is None:
+        max_pairs = pairs.shape[0] if pairs is not None else kp1.shape[0]
+    _check(lib().tdm_ransac_rt(None, _p(kp1), _p(kp2), _p(pairs), _p(num_pairs),
+                               c_int(max_pairs), ctypes.c_double(threshold),
+                               ctypes.c_long(max_trials), ctypes.c_uint64(seed), _p(rt),
+                               _p(inliers), _p(stats), _p(trial_counts), _stream(stream)))
+
+
+def estimate_rt(x1, x2, rt, stream=None) -> None:
+    """estimateRt (core/external/sfm/estimateRt.m): rt float64 [12]."""
+    _check(lib().tdm_estimate_rt(None, _p(x1), _p(x2), c_int(x1.shape[0]), _p(rt),
+                                 _stream(stream)))
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", c_size_t), ("max_inflight", c_int), ("keypoints", c_int),
+                ("max_pass_keypoints", c_int), ("voxel_size", c_float),
+                ("trunc_margin", c_float), ("padding", c_int),
+                ("ransac_threshold", ctypes.c_double), ("ransac_max_trials", ctypes.c_long)]
+
+
+class _RegRequest(ctypes.Structure):
+    _fields_ = [("points1", c_void_p), ("num_points1", c_int), ("points2", c_void_p),
+                ("num_points2", c_int), ("keypoints1", c_void_p), ("keypoints2", c_void_p),
+                ("num_keypoints", c_int), ("seed", ctypes.c_uint64)]
+
+
+class _RegResult(ctypes.Structure):
+    _fields_ = [("rt", ctypes.c_double * 12), ("status", c_int), ("num_matches", c_int),
+                ("num_inliers", c_int), ("trials", c_int), ("fallback", c_int),
+                ("pass_keypoints", c_int), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double)]
+
+
+class Server:
+    """tdm_server: fragment-pair registration requests with continuous
+    batching of descriptor passes and admission by a device-memory budget."""
+
+    def __init__(self, net: "Net", device: int = 0, **cfg):
+        c = ServerConfig()
+        lib().tdm_server_config_init(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.config = c
+        self._net = net  # the server drives it; keep it alive
+        h = c_void_p()
+        _check(lib().tdm_server_create(net._h, c_int(device), ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, points1: np.ndarray, points2: np.ndarray, keypoints1=None, keypoints2=None,
+               seed: int = 0) -> int:
+        p1 = np.ascontiguousarray(points1, dtype=np.float32).reshape(-1, 3)
+        p2 = np.ascontiguousarray(points2, dtype=np.float32).reshape(-1, 3)
+        r = _RegRequest(p1.ctypes.data, len(p1), p2.ctypes.data, len(p2), None, None, 0, seed)
+        keep = []
+        if keypoints1 is not None:
+            k1 = np.ascontiguousarray(keypoints1, dtype=np.int32)
+            k2 = np.ascontiguousarray(keypoints2, dtype=np.int32)
+            assert len(k1) == len(k2)
+            r.keypoints1, r.keypoints2, r.num_keypoints = k1.ctypes.data, k2.ctypes.data, len(k1)
+            keep = [k1, k2]
+        t = ctypes.c_uint64()
+        _check(lib().tdm_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
+        del keep
+        return int(t.value)
+
+    def wait(self, ticket: int) -> dict:
+        res = _RegResult()
+        _check(lib().tdm_server_wait(self._h, ctypes.c_uint64(ticket), ctypes.byref(res)))
+        return {"rt": np.array(res.rt[:]).reshape(3, 4), "status": res.status,
+                "num_matches": res.num_matches, "num_inliers": res.num_inliers,
+                "trials": res.trials, "fallback": res.fallback,
+                "pass_keypoints": res.pass_keypoints, "queue_ms": res.queue_ms,
+                "latency_ms": res.latency_ms}
+
+    def stats(self) -> dict:
+        s = (ctypes.c_longlong * 6)()
+        _check(lib().tdm_server_stats(self._h, s))
+        keys = ["peak_bytes", "budget", "max_inflight", "passes", "completed", "max_pass_requests"]
+        return dict(zip(keys, [int(v) for v in s]))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(lib().tdm_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def grid_table(grids, device="cuda:0"):
+    """Device table of tdm_grid_ref entries for (tdf tensor, dims) pairs; the
+    tensors must stay alive while the table is used."""
+    import torch
+
+    raw = np.zeros((len(grids), 3), dtype=np.int64)  # 24-byte entries
+    for i, (t, d) in enumerate(grids):
+        raw[i, 0] = t.data_ptr()
+        dims = np.array([d[0], d[1], d[2], 0], dtype=np.int32)
+        raw[i, 1:] = dims.view(np.int64)
+    return torch.from_numpy(raw).to(device)
+
+
 # ---- network --------------------------------------------------------------------
 
 
@@ -240,6 +372,12 @@ class Net:
         _check(lib().tdm_net_forward_grid(self._h, _p(tdf), c_int(dims[0]), c_int(dims[1]),
                                           c_int(dims[2]), _p(kp_grid), c_int(kp_grid.shape[0]),
                                           _p(feat), _stream(stream)))
+
+    def forward_grids(self, table, kp_grid, grid_index, feat, stream=None) -> None:
+        """table: device tensor from grid_table(); kp_grid int32 [n, 3] and
+        grid_index int32 [n] on the device."""
+        _check(lib().tdm_net_forward_grids(self._h, _p(table), _p(kp_grid), _p(grid_index),
+                                           c_int(kp_grid.shape[0]), _p(feat), _stream(stream)))
 
     def forward_layer(self, data, layer_index: int, out, stream=None) -> None:
         _check(lib().tdm_net_forward_layer(self._h, _p(data), c_int(data.shape[0]),

@@ -143,3 +143,50 @@ def keypoint_indices(n_points: int, count: int = 5000, seed: int = 4) -> np.ndar
     core/utils.hpp:15-20), from a seeded generator."""
     rng = np.random.default_rng(seed)
     return np.floor(rng.uniform(0.0, n_points - 1e-4, size=count)).astype(np.int64)
+
+
+_VIEW_CACHE: dict = {}
+
+
+def _view_points(view: int, scene_seed: int) -> np.ndarray:
+    """Noise-free back-projected points [N, 3] (camera frame) of one view."""
+    key = (view, scene_seed)
+    if key not in _VIEW_CACHE:
+        z = raycast_depth(camera_poses(50)[view], spheres(scene_seed))
+        v, u = np.nonzero(np.isfinite(z) & (z > 0) & (z < 6))
+        zz = z[v, u]
+        _VIEW_CACHE[key] = np.stack([(u - CX) / FX * zz, (v - CY) / FY * zz, zz], axis=1)
+    return _VIEW_CACHE[key]
+
+
+def fragment_pair(seed: int, n1: int | None = None, n2: int | None = None,
+                  overlap: float | None = None, max_angle: float = np.pi / 3,
+                  max_shift: float = 1.0, sigma: float = 0.002, scene_seed: int = 1):
+    """One fragment-pair registration request: two parts of a ray-cast view
+    that share `overlap` of each part's extent (default U[0.5, 0.8]), each
+    resampled to its point count (default U[20k, 500k]) with N(0, sigma^2)
+    noise; fragment 2 is then moved by a rigid motion (angle U[0, max_angle]
+    about a uniform axis, shift of length U[0, max_shift]).  Returns
+    (pts1, pts2, R, t) with pts1 = R pts2 + t on the shared surface."""
+    rng = np.random.default_rng(seed)
+    P = _view_points(int(rng.integers(0, 50)), scene_seed)
+    o = rng.uniform(0.5, 0.8) if overlap is None else overlap
+    a = 1.0 / (2.0 - o)  # each part spans a quantile range of width a; they share 2a - 1
+    key = P[:, 0]
+    lo, hi = np.quantile(key, [1.0 - a, a])
+    parts = [P[key <= hi], P[key >= lo]]
+    counts = [n1, n2]
+    out = []
+    for part, n in zip(parts, counts):
+        n = int(rng.integers(20_000, 500_001)) if n is None else n
+        idx = rng.choice(len(part), size=n, replace=len(part) < n)
+        out.append(part[idx] + rng.standard_normal((n, 3)) * sigma)
+    axis = rng.standard_normal(3)
+    axis /= np.linalg.norm(axis)
+    ang = rng.uniform(0.0, max_angle)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+    d = rng.standard_normal(3)
+    t = d / np.linalg.norm(d) * rng.uniform(0.0, max_shift)
+    pts2 = (out[1] - t) @ R  # R^T (x - t) for row vectors
+    return out[0].astype(np.float32), pts2.astype(np.float32), R, t

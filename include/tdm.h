@@ -196,6 +196,20 @@ int tdm_net_forward(tdm_net *net, const float *data, int num, float *feat, void 
 int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, int dim_z,
                          const int *kp_grid, int num, float *feat, void *stream);
 
+/* One TDF grid of a multi-grid pass: device pointer and extents. */
+typedef struct tdm_grid_ref {
+    const float *tdf;
+    int dim_x, dim_y, dim_z;
+    int reserved;
+} tdm_grid_ref;
+
+/* Descriptors for keypoints spread over several TDF grids in one pass (the
+ * per-fragment loop of core/demo.cu:195-231 batched across fragments):
+ * grids is a device table; keypoint b (grid coords kp_grid[3b..3b+2]) reads
+ * grid grid_index[b].  All pointers device. */
+int tdm_net_forward_grids(tdm_net *net, const tdm_grid_ref *grids, const int *kp_grid,
+                          const int *grid_index, int num, float *feat, void *stream);
+
 /* Output of convolution layer `layer_index` (after its ReLU, and after the
  * pool that follows it, if any) for `num` patches, as float NCDHW into out
  * (device).  For per-layer error measurements. */
@@ -211,6 +225,131 @@ int tdm_net_set_profiling(tdm_net *net, int enable);
  * layer since profiling was enabled.  Waits for the recorded events. */
 int tdm_net_layer_stats(tdm_net *net, int layer_index, double *total_ms, long *launches,
                         long *patches);
+
+/* ---- registration: matching and RANSAC ------------------------------------ */
+
+/* Nearest neighbour of every descriptor of A among B and of every descriptor
+ * of B among A, under Euclidean distance: the two knnsearch calls on
+ * KDTreeSearcher of core/demo.m:49-52 and register2Fragments.m:49-52
+ * (evaluation/geometric-registration).  Exact: the result is the argmin of
+ * the exact squared distance, ties to the lower index (0-based).  desc_a,
+ * desc_b: device, row-major num*dim floats, 16-byte aligned, dim a multiple
+ * of 4 up to 2048.  Outputs (device, any may be NULL but not both nn):
+ * nn_ab[num_a], nn_ba[num_b] indices; dist_ab/dist_ba the Euclidean
+ * distances (knnsearch's second output, fp64 sums rounded to float);
+ * refined[2] the number of queries of A and of B whose fp32 answer was not
+ * certified and that were searched again in fp64. */
+int tdm_match_nn(tdm_ctx *ctx, const float *desc_a, int num_a, const float *desc_b, int num_b,
+                 int dim, int *nn_ab, float *dist_ab, int *nn_ba, float *dist_ba, int *refined,
+                 void *stream);
+
+/* Mutually nearest pairs (core/demo.m:53-55): j of B is kept when
+ * nn_ab[nn_ba[j]] == j; pairs (device, capacity num_b*2 ints) receives
+ * (nn_ba[j], j) in ascending j, num_pairs (device, 1 int) their count. */
+int tdm_mutual_pairs(tdm_ctx *ctx, const int *nn_ab, int num_a, const int *nn_ba, int num_b,
+                     int *pairs, int *num_pairs, void *stream);
+
+/* ransacfitRt(x, threshold, 0) (core/external/sfm/ransacfitRt.m:20-64 with
+ * core/external/peter/ransac.m:113-231): the rigid [R|t] with x1 = R*x2 + t
+ * over pairs of kp1 (fragment 1) and kp2 (fragment 2), all fits fp64.
+ *   kp1, kp2   device, 3 floats per keypoint
+ *   pairs      device (i, j) index pairs into kp1/kp2, or NULL for (k, k)
+ *   num_pairs  device count of pairs (e.g. from tdm_mutual_pairs), or NULL
+ *   max_pairs  the host bound on that count (the count when num_pairs is NULL)
+ *   threshold  inlier distance (0.05 in the reference's callers)
+ *   max_trials the trial cap (50000 in ransac.m); p = 0.99 as there
+ *   seed       the trial samples come from splitmix64(seed, trial), not from
+ *              MATLAB's RandStream
+ * Outputs (device): rt 12 doubles, row-major 3x4; inliers (capacity
+ * max_pairs) the pair indices within threshold of the kept hypothesis, in
+ * ascending order; stats 4 ints = {inlier count, kept trial (-1 when no
+ * sampling ran), trials run, 1 if fewer than 3 pairs (rt = identity, the
+ * fallback of register2Fragments.m:61-64) else 0}; trial_counts (NULL or
+ * max_trials+1 ints) every trial's inlier count.  Three pairs are fitted
+ * directly (ransacfitRt.m:22-26). */
+int tdm_ransac_rt(tdm_ctx *ctx, const float *kp1, const float *kp2, const int *pairs,
+                  const int *num_pairs, int max_pairs, double threshold, long max_trials,
+                  uint64_t seed, double *rt, int *inliers, int *stats, int *trial_counts,
+                  void *stream);
+
+/* estimateRt (core/external/sfm/estimateRt.m, estimateRigidTransform.m:
+ * 43-72): least-squares [R|t] with x1 = R*x2 + t over num >= 3 pairs, by
+ * Horn's quaternion method in fp64.  x1, x2: device, num*3 floats; rt:
+ * device, 12 doubles, row-major 3x4. */
+int tdm_estimate_rt(tdm_ctx *ctx, const float *x1, const float *x2, int num, double *rt,
+                    void *stream);
+
+/* ---- registration serving ---------------------------------------------------------
+ *
+ * A request registers fragment 2 onto fragment 1, the pipeline of
+ * evaluation/geometric-registration/register2Fragments.m:1-65 (with the
+ * descriptors of core/demo.cu): point cloud -> TDF -> descriptors at
+ * keypoints -> mutual nearest neighbours -> ransacfitRt.  The server keeps
+ * requests' TDF grids, descriptors and matches resident in one device
+ * arena of `memory_budget` bytes and admits a queued request when its
+ * buffers fit and a slot is free.  Admitted requests run concurrently on
+ * their own streams; whenever the descriptor network is free, the keypoints
+ * of every request whose TDF grids are ready go into one multi-grid pass
+ * (continuous batching), so requests that arrive while others are on the
+ * GPU join the next pass.  Results do not depend on what else is in flight:
+ * every kernel is deterministic and per-patch results do not depend on the
+ * pass they run in.  One scheduler thread issues all GPU work. */
+typedef struct tdm_server tdm_server;
+
+typedef struct tdm_server_config {
+    size_t memory_budget;     /* device bytes for requests' buffers           */
+    int max_inflight;         /* requests admitted at once (streams)          */
+    int keypoints;            /* keypoints per fragment when not given        */
+    int max_pass_keypoints;   /* keypoints per descriptor pass                */
+    float voxel_size;         /* TDF voxel (core/demo.cu:97: 0.01)            */
+    float trunc_margin;       /* TDF truncation (core/demo.cu:98: 5 voxels)   */
+    int padding;              /* grid padding in voxels (core/demo.cu:99: 15) */
+    double ransac_threshold;  /* register2Fragments.m:59: 0.05               */
+    long ransac_max_trials;   /* ransac.m:121: 50000                          */
+} tdm_server_config;
+
+/* The defaults above; memory_budget 4 GiB, 8 slots, 5000 keypoints, passes
+ * of up to 20000 keypoints. */
+void tdm_server_config_init(tdm_server_config *cfg);
+
+typedef struct tdm_reg_request {
+    const float *points1;   /* host, num_points1*3 floats (fragment 1)      */
+    int num_points1;
+    const float *points2;   /* host, num_points2*3 floats (fragment 2)      */
+    int num_points2;
+    const int *keypoints1;  /* host point indices, or NULL: chosen from seed */
+    const int *keypoints2;
+    int num_keypoints;      /* when indices are given (0 = config value)     */
+    uint64_t seed;          /* keypoint choice and RANSAC samples            */
+} tdm_reg_request;
+
+typedef struct tdm_reg_result {
+    double rt[12];       /* row-major [R|t] with x1 = R x2 + t                */
+    int status;          /* TDM_OK or the error of this request               */
+    int num_matches;     /* mutually nearest keypoint pairs                   */
+    int num_inliers;     /* of the kept RANSAC model                          */
+    int trials;          /* RANSAC trials run                                 */
+    int fallback;        /* 1: fewer than 3 matches, rt = identity            */
+    int pass_keypoints;  /* keypoints in the descriptor pass it ran in        */
+    double queue_ms;     /* submit -> admitted                                */
+    double latency_ms;   /* submit -> result ready                            */
+} tdm_reg_result;
+
+/* Server on HIP device `device` using `net` (which it then drives
+ * exclusively until destroyed). */
+int tdm_server_create(tdm_net *net, int device, const tdm_server_config *cfg,
+                      tdm_server **out);
+/* Queues a request; the point arrays are copied before return.  Fails with
+ * TDM_ERR_NOMEM if the request alone exceeds the memory budget. */
+int tdm_server_submit(tdm_server *srv, const tdm_reg_request *req, uint64_t *ticket);
+/* Blocks until the request is done and returns its result (once). */
+int tdm_server_wait(tdm_server *srv, uint64_t ticket, tdm_reg_result *out);
+/* stats[0] peak arena bytes in use, [1] budget, [2] most requests in flight
+ * at once, [3] descriptor passes, [4] requests completed, [5] most requests
+ * in one pass. */
+int tdm_server_stats(tdm_server *srv, long long stats[6]);
+/* Finishes the queued and in-flight requests, then frees everything. */
+int tdm_server_destroy(tdm_server *srv);
 
 /* ---- file formats (host) --------------------------------------------------- */
 

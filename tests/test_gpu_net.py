@@ -139,3 +139,38 @@ def test_fp32_vs_oracle_fp32_loop(torch_gpu, net_files, patches):
     feat = torch_gpu.empty(2, 512, dtype=torch_gpu.float32, device="cuda:0")
     net.forward(torch_gpu.from_numpy(p).to("cuda:0"), feat)
     assert rel_l2(feat.cpu().numpy(), ref) <= FP32_TOL
+
+
+def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
+    """Keypoints of three fragments interleaved in one multi-grid pass give
+    the same descriptors, bit for bit, as one pass per grid."""
+    torch = torch_gpu
+    js, wt, _ = net_files
+    vs = np.float32(0.01)
+    grids, kps = [], []
+    for k, (n, view) in enumerate([(30_000, 0), (45_000, 10), (20_000, 25)]):
+        cloud = synth.fragment(n, view=view, seed=40 + k)
+        dims, origin = oracle.grid_params(cloud, vs, 15)
+        occ = oracle.occupancy(cloud, dims, origin, vs)
+        tdf = oracle.compute_tdf(occ, vs, np.float32(0.05))
+        idx = synth.keypoint_indices(len(cloud), 5 + 2 * k, seed=50 + k)
+        kg = oracle.keypoint_grid(cloud[idx], origin, vs)
+        grids.append((torch.from_numpy(tdf).to("cuda:0"), dims))
+        kps.append(kg)
+    order = np.concatenate([np.full(len(k), i) for i, k in enumerate(kps)])
+    kp_all = np.concatenate(kps)
+    perm = np.random.default_rng(3).permutation(len(order))
+    table = tdm.grid_table(grids)
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
+        net = tdm.Net(js, prec, max_batch=8)
+        net.load_weights(wt)
+        per = []
+        for (t, dims), kg in zip(grids, kps):
+            f = torch.empty(len(kg), 512, dtype=torch.float32, device="cuda:0")
+            net.forward_grid(t, dims, torch.from_numpy(kg).to("cuda:0"), f)
+            per.append(f)
+        want = torch.cat(per)[torch.from_numpy(perm).to("cuda:0")]
+        got = torch.empty_like(want)
+        net.forward_grids(table, torch.from_numpy(np.ascontiguousarray(kp_all[perm])).to("cuda:0"),
+                          torch.from_numpy(order[perm].astype(np.int32)).to("cuda:0"), got)
+        assert torch.equal(got, want), prec
