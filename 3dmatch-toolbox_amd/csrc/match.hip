@@ -887,7 +887,6 @@ int tdm_match_nn(tdm_ctx *ctx, const float *desc_a, int num_a, const float *desc
     const size_t na = num_a, nb = num_b;
     const size_t bytes = tdm::match_scratch_bytes(num_a, num_b);
     void *base = nullptr;
-    std::lock_guard<std::mutex> lock(ctx->mu);
     TDM_TRY(tdm::ctx_scratch(ctx, 2, bytes, &base, s));
     unsigned char *cur = static_cast<unsigned char *>(base);
     float4 *part_a = carve<float4>(cur, (size_t)tiles_b * na);
@@ -965,7 +964,6 @@ int tdm_ransac_rt(tdm_ctx *ctx, const float *kp1, const float *kp2, const int *p
     hipStream_t s = tdm::as_stream(stream);
     const long T = max_trials + 1;  // ransac.m runs trials until trialcount > maxTrials
     void *base = nullptr;
-    std::lock_guard<std::mutex> lock(ctx->mu);
     TDM_TRY(tdm::ctx_scratch(ctx, 3, tdm::ransac_scratch_bytes(max_trials), &base, s));
     unsigned char *cur = static_cast<unsigned char *>(base);
     double *hyp = carve<double>(cur, 12 * (size_t)T);
@@ -993,7 +991,6 @@ int tdm_estimate_rt(tdm_ctx *ctx, const float *x1, const float *x2, int num, dou
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     hipStream_t s = tdm::as_stream(stream);
     void *base = nullptr;
-    std::lock_guard<std::mutex> lock(ctx->mu);
     TDM_TRY(tdm::ctx_scratch(ctx, 3, carved(4, 4) + carved(num, 4), &base, s));
     unsigned char *cur = static_cast<unsigned char *>(base);
     int *stats = carve<int>(cur, 4);

@@ -243,6 +243,98 @@ int check_grid(int H, int W, int dx, int dy, int dz, float vs, float trunc) {
     return TDM_OK;
 }
 
+// ---- surface points (SaveVoxelGrid2SurfacePointCloud, depth-fusion/utils.hpp:14-56) ----
+//
+// Stream compaction in linear-index order: a workgroup owns a chunk of
+// kSurfChunk voxels read as 16 coalesced sweeps of 256; pass 1 counts the
+// chunk's surface voxels, one workgroup scans the counts, pass 2 recounts
+// sweep by sweep (ballots give each voxel its rank) and writes the points.
+constexpr int kSurfSweeps = 16;
+constexpr long kSurfChunk = 256L * kSurfSweeps;
+
+__device__ inline bool surface_voxel(const float *__restrict__ tsdf,
+                                     const float *__restrict__ weight, long i, float tt,
+                                     float wt) {
+    return fabsf(tsdf[i]) < tt && weight[i] > wt;
+}
+
+__global__ __launch_bounds__(256) void surface_count_kernel(const float *__restrict__ tsdf,
+                                                            const float *__restrict__ weight,
+                                                            long n, float tt, float wt,
+                                                            long long *__restrict__ counts) {
+    __shared__ int wsum[4];
+    const long base = (long)blockIdx.x * kSurfChunk;
+    int c = 0;
+    for (int k = 0; k < kSurfSweeps; ++k) {
+        const long i = base + 256L * k + threadIdx.x;
+        c += (i < n && surface_voxel(tsdf, weight, i, tt, wt)) ? 1 : 0;
+    }
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Exclusive scan of the chunk counts in place; total into counts[nb].
+__global__ __launch_bounds__(1024) void surface_scan_kernel(long long *__restrict__ counts,
+                                                            long nb) {
+    __shared__ long long part[1024];
+    const long per = (nb + 1023) / 1024;
+    const long b0 = min(nb, (long)threadIdx.x * per), b1 = min(nb, b0 + per);
+    long long s = 0;
+    for (long b = b0; b < b1; ++b) s += counts[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
+    for (long b = b0; b < b1; ++b) {
+        const long long c = counts[b];
+        counts[b] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) counts[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void surface_write_kernel(
+    const float *__restrict__ tsdf, const float *__restrict__ weight, long n, float tt, float wt,
+    int dx, int dy, float ox, float oy, float oz, float vs, const long long *__restrict__ offsets,
+    float *__restrict__ pts, long long capacity) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long base = (long)blockIdx.x * kSurfChunk;
+    const long plane = (long)dx * dy;
+    long long at = offsets[blockIdx.x];
+    for (int k = 0; k < kSurfSweeps; ++k) {
+        const long i = base + 256L * k + threadIdx.x;
+        const bool keep = i < n && surface_voxel(tsdf, weight, i, tt, wt);
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        long long off = at;
+        int total = 0;
+        for (int w = 0; w < 4; ++w) {
+            if (w < wave) off += wsum[w];
+            total += wsum[w];
+        }
+        off += __popcll(m & ((1ull << lane) - 1ull));
+        if (keep && off < capacity) {
+            // integer decode (the reference's floor of an int quotient)
+            const long z = i / plane, r = i - z * plane;
+            const long y = r / dx, x = r - y * dx;
+            pts[3 * off + 0] = ox + (float)x * vs;
+            pts[3 * off + 1] = oy + (float)y * vs;
+            pts[3 * off + 2] = oz + (float)z * vs;
+        }
+        at += total;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,6 +400,41 @@ int tdm_depth_from_mm(tdm_ctx *ctx, const uint16_t *depth_mm, size_t n, float *d
     depth_from_mm_kernel<<<tdm::blocks_for(n, 256), 256, 0, tdm::as_stream(stream)>>>(
         depth_mm, depth_m, n);
     TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_surface_points(tdm_ctx *ctx, const float *tsdf, const float *weight, const int dims[3],
+                       const float origin[3], float voxel_size, float tsdf_thresh,
+                       float weight_thresh, float *pts, size_t capacity, size_t *num_points,
+                       void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(tsdf && weight && dims && origin && num_points, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_CHECK(pts || capacity == 0, TDM_ERR_INVALID_ARG, "no output buffer");
+    TDM_CHECK(dims[0] > 0 && dims[1] > 0 && dims[2] > 0, TDM_ERR_INVALID_ARG, "bad dims");
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    const long n = (long)dims[0] * dims[1] * dims[2];
+    const long nb = (n + kSurfChunk - 1) / kSurfChunk;
+    TDM_CHECK(nb < (1L << 31), TDM_ERR_INVALID_ARG, "grid too large");
+    void *scratch = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 1, sizeof(long long) * (nb + 1), &scratch, s));
+    long long *counts = static_cast<long long *>(scratch);
+    surface_count_kernel<<<(unsigned)nb, 256, 0, s>>>(tsdf, weight, n, tsdf_thresh,
+                                                       weight_thresh, counts);
+    TDM_LAUNCHED();
+    surface_scan_kernel<<<1, 1024, 0, s>>>(counts, nb);
+    TDM_LAUNCHED();
+    if (capacity > 0) {
+        surface_write_kernel<<<(unsigned)nb, 256, 0, s>>>(
+            tsdf, weight, n, tsdf_thresh, weight_thresh, dims[0], dims[1], origin[0], origin[1],
+            origin[2], voxel_size, counts, pts, (long long)capacity);
+        TDM_LAUNCHED();
+    }
+    long long total = 0;
+    TDM_HIP(hipMemcpyAsync(&total, counts + nb, sizeof total, hipMemcpyDeviceToHost, s));
+    TDM_HIP(hipStreamSynchronize(s));
+    *num_points = (size_t)total;
     return TDM_OK;
 }
 

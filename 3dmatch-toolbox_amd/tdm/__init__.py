@@ -125,6 +125,17 @@ def integrate_frames(cam_K, cam2base, depths, dims, origin, voxel_size, trunc, t
                                       _stream(stream)))
 
 
+def surface_points(tsdf, weight, dims, origin, voxel_size, pts=None, capacity=0,
+                   tsdf_thresh=0.2, weight_thresh=1.0, stream=None) -> int:
+    """GPU surface extraction (depth-fusion/utils.hpp:14-56) into pts
+    [capacity, 3]; returns the total point count (synchronizes)."""
+    n = c_size_t()
+    _check(lib().tdm_surface_points(None, _p(tsdf), _p(weight), _int3(dims), _float3(origin),
+                                    _f(voxel_size), _f(tsdf_thresh), _f(weight_thresh), _p(pts),
+                                    c_size_t(capacity), ctypes.byref(n), _stream(stream)))
+    return int(n.value)
+
+
 def depth_from_mm(mm, out, stream=None) -> None:
     _check(lib().tdm_depth_from_mm(None, _p(mm), c_size_t(mm.numel()), _p(out), _stream(stream)))
 

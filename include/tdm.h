@@ -105,6 +105,19 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
                          float origin_z, float voxel_size, float trunc_margin, float *tsdf,
                          float *weight, void *stream);
 
+/* Surface points of a fused grid on the GPU: what
+ * SaveVoxelGrid2SurfacePointCloud (depth-fusion/utils.hpp:14-56) writes to
+ * tsdf.ply -- every voxel with |tsdf| < tsdf_thresh and weight >
+ * weight_thresh (0.2 and 1 at depth-fusion/demo.cu:192-194), in linear
+ * index order, as origin + (float)index * voxel_size per axis.  tsdf,
+ * weight: device grids; pts: device, capacity*3 floats (the first
+ * `capacity` points are written); *num_points (host) receives the total.
+ * Synchronizes `stream`. */
+int tdm_surface_points(tdm_ctx *ctx, const float *tsdf, const float *weight, const int dims[3],
+                       const float origin[3], float voxel_size, float tsdf_thresh,
+                       float weight_thresh, float *pts, size_t capacity, size_t *num_points,
+                       void *stream);
+
 /* u16 millimetre depth (device, n values) -> metres with values above 6 m set
  * to 0 (depth-fusion/utils.hpp:74-86 after the PNG decode). */
 int tdm_depth_from_mm(tdm_ctx *ctx, const uint16_t *depth_mm, size_t n, float *depth_m,

@@ -233,3 +233,21 @@ def compute_tdf_range(occ: np.ndarray, tdf: np.ndarray, v0: int, v1: int, voxel_
     dz, dy, dx = occ.shape
     lib().orc_compute_tdf_exact_range(_ptr(occ), _ptr(tdf), c_int(dx), c_int(dy), c_int(dz),
                                       c_float(voxel_size), c_float(trunc), c_long(v0), c_long(v1))
+
+
+def surface_points(tsdf: np.ndarray, weight: np.ndarray, origin, voxel_size, tsdf_thresh=0.2,
+                   weight_thresh=1.0) -> np.ndarray:
+    """SaveVoxelGrid2SurfacePointCloud (depth-fusion/utils.hpp:14-56): the
+    voxels with |tsdf| < tsdf_thresh and weight > weight_thresh in linear
+    index order, as origin + (float)index * voxel_size (float32, no FMA)."""
+    dz, dy, dx = tsdf.shape
+    t = tsdf.reshape(-1)
+    w = weight.reshape(-1)
+    idx = np.nonzero((np.abs(t) < np.float32(tsdf_thresh)) & (w > np.float32(weight_thresh)))[0]
+    z = idx // (dx * dy)
+    y = (idx - z * dx * dy) // dx
+    x = idx - z * dx * dy - y * dx
+    o = np.asarray(origin, np.float32)
+    vs = np.float32(voxel_size)
+    return np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
+                     o[2] + z.astype(np.float32) * vs], axis=1)

@@ -151,3 +151,36 @@ def test_plane_closed_form(torch_gpu):
         else:
             assert w[z, 4, 4] == 2
             assert t[z, 4, 4] == np.float32(min(np.float32(1.0), np.float32(diff / trunc)))
+
+
+def surface_gpu(torch, tsdf, weight, dims, origin, vs, capacity=None):
+    dt = torch.from_numpy(np.ascontiguousarray(tsdf)).to("cuda:0")
+    dw = torch.from_numpy(np.ascontiguousarray(weight)).to("cuda:0")
+    n = tdm.surface_points(dt, dw, dims, origin, vs, None, 0)  # count only
+    cap = n if capacity is None else capacity
+    pts = torch.full((max(cap, 1), 3), -7.0, dtype=torch.float32, device="cuda:0")
+    n2 = tdm.surface_points(dt, dw, dims, origin, vs, pts, cap)
+    assert n2 == n
+    return n, pts[:cap].cpu().numpy()
+
+
+def test_surface_points_bit_exact(torch_gpu, synth_frames):
+    """tsdf.ply's point set (depth-fusion/utils.hpp:14-56) from the GPU
+    compaction equals the oracle's, bit for bit and in order; a smaller
+    capacity gets the same leading points."""
+    depths, poses, K = synth_frames
+    dims = (160, 120, 136)
+    origin = np.array([-0.8, -0.6, 0.9], np.float32)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    t, w = _gpu_fuse(torch_gpu, K, poses, depths, dims, origin, vs, trunc, True)
+    ref = oracle.surface_points(t, w, origin, vs)
+    assert len(ref) > 1000
+    n, pts = surface_gpu(torch_gpu, t, w, dims, origin, vs)
+    assert n == len(ref)
+    assert np.array_equal(pts.view(np.uint32), ref.view(np.uint32))
+    n, pts = surface_gpu(torch_gpu, t, w, dims, origin, vs, capacity=len(ref) // 3)
+    assert n == len(ref)
+    assert np.array_equal(pts.view(np.uint32), ref[: len(ref) // 3].view(np.uint32))
+    # nothing passes: empty set
+    n, _ = surface_gpu(torch_gpu, np.ones_like(t), w, dims, origin, vs)
+    assert n == 0

@@ -129,3 +129,30 @@ def test_weight_invariant_and_running_average():
     assert np.array_equal(w2, 2 * w1)
     assert np.array_equal(t2, t1)  # (d*1 + d)/2 == d exactly
     assert set(np.unique(w1)) <= {0.0, 1.0}
+
+
+def test_surface_points_oracle_matches_literal_loop():
+    """The vectorised surface extraction against a literal restatement of the
+    reference loop (depth-fusion/utils.hpp:44-56) on a small grid."""
+    rng = np.random.default_rng(11)
+    dims = (7, 5, 6)
+    tsdf = rng.uniform(-1, 1, (dims[2], dims[1], dims[0])).astype(np.float32)
+    weight = rng.integers(0, 4, tsdf.shape).astype(np.float32)
+    tsdf[0, 0, 0] = np.float32(0.2)  # boundary: not < 0.2
+    weight[0, 0, 1] = np.float32(1.0)  # boundary: not > 1
+    origin = np.array([-0.31, 0.07, 1.13], np.float32)
+    vs = np.float32(0.013)
+    got = oracle.surface_points(tsdf, weight, origin, vs)
+    t, w = tsdf.reshape(-1), weight.reshape(-1)
+    want = []
+    for i in range(dims[0] * dims[1] * dims[2]):
+        if abs(float(t[i])) < 0.2 and float(w[i]) > 1.0:
+            z = i // (dims[0] * dims[1])
+            y = (i - z * dims[0] * dims[1]) // dims[0]
+            x = i - z * dims[0] * dims[1] - y * dims[0]
+            want.append([np.float32(origin[0]) + np.float32(x) * vs,
+                         np.float32(origin[1]) + np.float32(y) * vs,
+                         np.float32(origin[2]) + np.float32(z) * vs])
+    want = np.array(want, np.float32)
+    assert len(want) > 10
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
