@@ -399,29 +399,40 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
     }
 }
 
-// LDS-staged implicit GEMM.  A workgroup computes a 128 x BN tile with four
-// waves in a 2 x 2 grid (wave tile 64 x BN/2).  Each stage is 128 bytes of k
-// per row -- one tap and 32 (fp32) or 64 (bf16) input channels: the 256
-// threads copy the stage's 128 + BN rows from global memory into registers
-// one stage ahead, then into the other half of a double-buffered LDS tile
-// (rows padded to 144 bytes against bank conflicts); the waves read their
-// fragments from LDS, so each A row is fetched once for BN/2-column halves
-// of two waves and each weight row once for two row halves.
+// LDS-staged implicit GEMM.  A workgroup computes a BM x BN tile with four
+// waves in a WMW x (4/WMW) grid, each wave owning a WTM x WTN sub-tile
+// (WTM/16 x WTN/16 MFMA blocks).  Each stage is 128 bytes of k per row -- one
+// tap and 32 (fp32) or 64 (16-bit) input channels: the 256 threads copy the
+// stage's BM + BN rows from global memory into registers one stage ahead,
+// then into the other half of a double-buffered LDS tile (rows padded to 144
+// bytes against bank conflicts); the waves read their fragments from LDS.
+// Larger wave tiles read fewer LDS bytes per MFMA; larger workgroup tiles
+// fetch each A row for more columns.  Every variant feeds each output the
+// same k in the same order (lane group g reads bytes 16g + 64h of a stage),
+// so all variants give bit-identical results and can be picked by timing.
 constexpr int kStageBytes = 128;
 constexpr int kLdsRow = 144;
 
-template <typename T, typename OT, int BN, bool POOL, bool SPLIT>
-__global__ __launch_bounds__(256, 2) void conv_lds_kernel(ConvArgs p) {
-    constexpr int BM = 128;
-    constexpr int WTN = BN / 2;
-    constexpr int IB = 4, JB = WTN / 16;
+template <int WMW, int WTM, int WTN>
+struct Tile {
+    static constexpr int WNW = 4 / WMW;
+    static constexpr int BM = WMW * WTM, BN = WNW * WTN;
+    static constexpr int LDS = 2 * (BM + BN) * kLdsRow;
+    static constexpr int OCC = LDS <= 80 * 1024 ? 2 : 1;  // workgroups per CU
+};
+
+template <typename T, typename OT, int WMW, int WTM, int WTN, bool POOL, bool SPLIT>
+__global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kernel(ConvArgs p) {
+    using TL = Tile<WMW, WTM, WTN>;
+    constexpr int BM = TL::BM, BN = TL::BN, WNW = TL::WNW;
+    constexpr int IB = WTM / 16, JB = WTN / 16;
     constexpr int SK = kStageBytes / (int)sizeof(T);  // k per stage
     constexpr int A_PT = BM * 8 / 256;                // 16-byte pieces per thread
     constexpr int B_PT = BN * 8 / 256;
     __shared__ __attribute__((aligned(16))) unsigned char lds[2][(BM + BN) * kLdsRow];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WNW, wn = wave % WNW;
     unsigned tx, ty, tz;
     xcd_tile(tx, ty, tz);
     const long mb = (long)tx * BM;
@@ -495,7 +506,7 @@ __global__ __launch_bounds__(256, 2) void conv_lds_kernel(ConvArgs p) {
             const int buf = (s - s0) & 1;
             const bool more = s + 1 < s1;
             if (more) fetch(s + 1);
-            const unsigned char *A = &lds[buf][(wm * 64 + r16) * kLdsRow + g * 16];
+            const unsigned char *A = &lds[buf][(wm * WTM + r16) * kLdsRow + g * 16];
             const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -512,7 +523,7 @@ __global__ __launch_bounds__(256, 2) void conv_lds_kernel(ConvArgs p) {
             __syncthreads();
         }
     }
-    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * 64, nb + wn * WTN, lane, tz);
+    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
 }
 
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
@@ -758,7 +769,8 @@ __global__ void maxpool_kernel(const T *__restrict__ in, long batch, int C, int 
 
 template <typename T, typename OT>
 int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void *w,
-                  const float *bias, void *out, int splits, float *partial, hipStream_t s) {
+                  const float *bias, void *out, int splits, float *partial, int variant,
+                  hipStream_t s) {
     ConvArgs p;
     p.in = in;
     p.w = w;
@@ -776,19 +788,27 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     p.M = (long)batch * d.OD * d.OD * d.OD;
     const bool lds = tdm::conv_uses_lds(dtype_of<T>(), d);
     if (lds) {
-        const int bn = d.N % 128 == 0 ? 128 : 64;
-        dim3 grid((unsigned)((p.M + 127) / 128), d.N / bn, splits);
-#define TDM_CONV_LDS(BN)                                                                      \
+        int v = variant;
+        if (v < 0 || !tdm::conv_variant_valid(v, d)) v = tdm::conv_default_variant(d);
+#define TDM_CONV_LDS(WMW, WTM, WTN)                                                           \
     do {                                                                                      \
+        using TL = Tile<WMW, WTM, WTN>;                                                       \
+        dim3 grid((unsigned)((p.M + TL::BM - 1) / TL::BM), d.N / TL::BN, splits);             \
         if (splits > 1)                                                                       \
-            conv_lds_kernel<T, OT, BN, false, true><<<grid, 256, 0, s>>>(p);                  \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, false, true><<<grid, 256, 0, s>>>(p);       \
         else if (d.pool)                                                                      \
-            conv_lds_kernel<T, OT, BN, true, false><<<grid, 256, 0, s>>>(p);                  \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, true, false><<<grid, 256, 0, s>>>(p);       \
         else                                                                                  \
-            conv_lds_kernel<T, OT, BN, false, false><<<grid, 256, 0, s>>>(p);                 \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, false, false><<<grid, 256, 0, s>>>(p);      \
     } while (0)
-        if (bn == 128) TDM_CONV_LDS(128);
-        else TDM_CONV_LDS(64);
+        switch (v) {
+        case 0: TDM_CONV_LDS(2, 64, 32); break;
+        case 1: TDM_CONV_LDS(2, 64, 64); break;
+        case 2: TDM_CONV_LDS(4, 64, 64); break;
+        case 3: TDM_CONV_LDS(2, 128, 32); break;
+        case 4: TDM_CONV_LDS(2, 128, 64); break;
+        default: TDM_CONV_LDS(1, 64, 64); break;
+        }
 #undef TDM_CONV_LDS
     } else {
         const int waves_n = d.N % 256 == 0 ? 4 : d.N % 128 == 0 ? 2 : 1;
@@ -842,6 +862,25 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
     return taps * d.C / (dt == DType::F32 ? 16 : 32);
 }
 
+// Tile variants of the LDS kernel: {waves along m, wave tile m, wave tile n}.
+namespace {
+constexpr int kVariants[kConvVariants][3] = {{2, 64, 32}, {2, 64, 64}, {4, 64, 64},
+                                             {2, 128, 32}, {2, 128, 64}, {1, 64, 64}};
+}
+
+bool conv_variant_valid(int v, const ConvDesc &d) {
+    if (v < 0 || v >= kConvVariants) return false;
+    const int bn = (4 / kVariants[v][0]) * kVariants[v][2];
+    return d.N % bn == 0;
+}
+
+int conv_default_variant(const ConvDesc &d) { return d.N % 128 == 0 ? 1 : 0; }
+
+void conv_variant_tile(int v, int *bm, int *bn) {
+    *bm = kVariants[v][0] * kVariants[v][1];
+    *bn = (4 / kVariants[v][0]) * kVariants[v][2];
+}
+
 uint16_t float_to_f16(float f) {
     const _Float16 h = (_Float16)f;  // IEEE conversion, round to nearest even
     uint16_t u;
@@ -859,7 +898,7 @@ uint16_t float_to_bf16(float f) {
 
 int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void *in,
                 const void *w, const float *bias, void *out, int splits, float *partial,
-                hipStream_t s) {
+                int variant, hipStream_t s) {
     TDM_CHECK(d.C % 32 == 0 && d.N % 64 == 0, TDM_ERR_UNSUPPORTED,
               "convolution with %d->%d channels (need multiples of 32 and 64)", d.C, d.N);
     TDM_CHECK(d.ksize == 3 || d.ksize == 1, TDM_ERR_UNSUPPORTED, "window %d", d.ksize);
@@ -868,13 +907,13 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
     TDM_CHECK(splits >= 1 && splits <= 65535 && (splits == 1 || partial), TDM_ERR_INVALID_ARG,
               "bad split");
     if (batch <= 0) return TDM_OK;
-    if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, s);
+    if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
     if (dt == DType::F16) {
-        if (out_f32) return conv_dispatch<f16_t, float>(d, batch, in, w, bias, out, splits, partial, s);
-        return conv_dispatch<f16_t, f16_t>(d, batch, in, w, bias, out, splits, partial, s);
+        if (out_f32) return conv_dispatch<f16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
+        return conv_dispatch<f16_t, f16_t>(d, batch, in, w, bias, out, splits, partial, variant, s);
     }
-    if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, s);
-    return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, s);
+    if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
+    return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, variant, s);
 }
 
 int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool relu, int batch,

@@ -34,9 +34,18 @@ int conv_reduction_steps(DType dt, const ConvDesc &d);
 // Launches the convolution for `batch` cubes.  out is NDHWC with the output
 // type: float when out_f32, else the storage type `dt`.  `splits` > 1 splits
 // the reduction and needs `partial` (splits*rows*N floats); no pooling then.
+// `variant` picks the LDS kernel's tile shape (-1: the default); every
+// variant gives bit-identical results.
 int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void *in,
                 const void *w, const float *bias, void *out, int splits, float *partial,
-                hipStream_t s);
+                int variant, hipStream_t s);
+
+// Tile variants of the LDS kernel (0 .. kConvVariants-1): whether one fits
+// a layer (its N is a multiple of the tile's BN), the default, and the tile.
+constexpr int kConvVariants = 6;
+bool conv_variant_valid(int v, const ConvDesc &d);
+int conv_default_variant(const ConvDesc &d);
+void conv_variant_tile(int v, int *bm, int *bn);
 
 // One TDF grid of a multi-grid pass (device table entry; the layout of
 // tdm_grid_ref).

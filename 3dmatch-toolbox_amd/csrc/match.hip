@@ -79,6 +79,11 @@ __device__ inline float comp(const uint4 &v, int s) {
     return __uint_as_float(u);
 }
 
+__device__ inline double int_to_dbits(int i) {
+    return __longlong_as_double((long long)(unsigned)i);
+}
+__device__ inline int dbits_to_int(double d) { return (int)__double_as_longlong(d); }
+
 // Squared length of every descriptor, summed in fp64 (one wave per row):
 // sq = float(|x|^2) feeds the fp32 distances, len = float(|x|) the error
 // bound, and maxlen the largest len of the set (as float bits).
@@ -263,15 +268,22 @@ __global__ __launch_bounds__(256, 2) void dist_top2_kernel(
 
 // Merges a query's tile partials and certifies its fp32 argmin.
 //
-// Error bound of one fp32 distance d~ = fl(fl(|q|^2) + fl(|c|^2) - 2 fl(q.c))
-// against the exact |q - c|^2, with u = 2^-24 and K = dim: the dot product's
-// error is at most K u |q||c| for any summation order; the two rounded
-// squared lengths and the two final operations add at most
-// 4 u (|q| + |c|)^2.  The bound used is eight times the first term plus the
-// second, with |c| replaced by the largest length of the candidate set:
-//   E = 8 K u |q| cmax + 4 u (|q| + cmax)^2.
-// Both the best and any rival carry at most E, so a gap above 2E proves the
-// fp32 winner is the exact winner.  Other queries go on the refine list.
+// The fp32 distances are taken between centred descriptors q' = fl(q - m),
+// c' = fl(c - m) for one float vector m (the mean of both sets): distances
+// do not change under a common shift, and the shift removes the common
+// component that dominates the lengths, so the bounds below shrink with it.
+// With u = 2^-24 and K = dim:
+//  - centring: |q' - (q - m)| <= u(1+u)|q'| per vector, so
+//    | |q'-c'| - |q-c| | <= eta = u(1+u)(|q'| + |c'|);
+//  - the fp32 distance d~ = fl(fl(|q'|^2) + fl(|c'|^2) - 2 fl(q'.c')) is
+//    within E = 8 K u |q'||c'| + 4 u (|q'| + |c'|)^2 of |q'-c'|^2 (eight
+//    times the summation-order-free dot product bound K u |q'||c'|, plus the
+//    rounded lengths and the final two operations).
+// |c'| is replaced by the largest centred length of the candidate set.  Every
+// rival has d~ >= second, so its exact distance is at least
+// sqrt(max(second - E, 0)) - eta, while the best's is at most
+// sqrt(best + E) + eta; when the first exceeds the second, the fp32 winner is
+// the exact winner.  Other queries go on the refine list.
 __global__ void nn_select_kernel(const float4 *__restrict__ part, int nparts, int n, int dim,
                                  const float *__restrict__ len,
                                  const unsigned *__restrict__ maxlen_other,
@@ -285,9 +297,16 @@ __global__ void nn_select_kernel(const float4 *__restrict__ part, int nparts, in
         t = top2_merge(t, {v.x, __float_as_int(v.y), v.z});
     }
     const double u = 5.9604644775390625e-8;  // 2^-24
-    const double q = len[i], c = __uint_as_float(*maxlen_other);
-    const double margin = 8.0 * dim * u * q * c + 4.0 * u * (q + c) * (q + c);
-    const bool sure = t.idx != INT_MAX && (double)t.second - (double)t.best > 2.0 * margin;
+    // lengths are fp64 sums rounded to float: inflate by a few ulps
+    const double q = len[i] * (1.0 + 4 * u), c = __uint_as_float(*maxlen_other) * (1.0 + 4 * u);
+    const double E = 8.0 * dim * u * q * c + 4.0 * u * (q + c) * (q + c);
+    const double eta = u * (1.0 + u) * (q + c);
+    bool sure = false;
+    if (t.idx != INT_MAX) {
+        const double lo = sqrt(fmax((double)t.second - E, 0.0)) - eta;
+        const double hi = sqrt((double)t.best + E) + eta;
+        sure = lo > hi;
+    }
     if (sure) {
         nn[i] = t.idx;
     } else {
@@ -296,11 +315,41 @@ __global__ void nn_select_kernel(const float4 *__restrict__ part, int nparts, in
     }
 }
 
-// fp64 squared distance between query q (LDS doubles or the float row) and
-// descriptor r: lane-strided sums folded by a butterfly, so every lane holds
-// the same value, and a pair gets the same value wherever this is used.
-template <typename QT>
-__device__ inline double sqdist64(const QT *q, const float *__restrict__ r, int dim, int lane) {
+// Column sums of rows [chunk*rows, (chunk+1)*rows) of X in fp64, for the
+// centring mean; part[chunk][k].
+__global__ __launch_bounds__(256) void colsum_kernel(const float *__restrict__ X, int n, int dim,
+                                                     int rows, double *__restrict__ part) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= dim) return;
+    const long r0 = (long)blockIdx.y * rows, r1 = min((long)n, r0 + rows);
+    double s = 0.0;
+    for (long r = r0; r < r1; ++r) s += X[r * dim + k];
+    part[(long)blockIdx.y * dim + k] = s;
+}
+
+// mean[k] = float(sum of both sets' partials / (na + nb)), partials in order.
+__global__ __launch_bounds__(256) void mean_kernel(const double *__restrict__ part, int nparts,
+                                                   int dim, double count,
+                                                   float *__restrict__ mean) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= dim) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(long)p * dim + k];
+    mean[k] = (float)(s / count);
+}
+
+// Y = fl(X - mean) row by row.
+__global__ __launch_bounds__(256) void center_kernel(const float *__restrict__ X, long total,
+                                                     int dim, const float *__restrict__ mean,
+                                                     float *__restrict__ Y) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long)gridDim.x * 256)
+        Y[i] = X[i] - mean[i % dim];
+}
+
+// fp64 squared distance between query q and descriptor r: lane-strided sums
+// folded by a butterfly, so every lane holds the same value.
+__device__ inline double sqdist64(const float *__restrict__ q, const float *__restrict__ r,
+                                  int dim, int lane) {
     double s = 0.0;
     for (int k = lane; k < dim; k += 64) {
         const double d = (double)q[k] - (double)r[k];
@@ -311,49 +360,106 @@ __device__ inline double sqdist64(const QT *q, const float *__restrict__ r, int 
     return s;
 }
 
-// Exact search for the queries on the refine list: four waves split the
-// candidates; the lowest (distance, index) wins.  Workgroups stride over the
-// list, whose length is read on the device.
-__global__ __launch_bounds__(256) void nn_refine_kernel(const float *__restrict__ Q,
-                                                        const float *__restrict__ R, int nr,
-                                                        int dim, const int *__restrict__ todo,
-                                                        const int *__restrict__ todo_count,
-                                                        int *__restrict__ nn) {
-    __shared__ double q[kMaxDim];
-    __shared__ double wbest[4];
-    __shared__ int widx[4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// Exact search for the queries on the refine list, as 64 x 64 tiles of
+// (query, candidate) pairs: each pair's squared distance is the fp64 sum of
+// (q_k - c_k)^2 in k order, so its value does not depend on the tiling.  The
+// tiles stage 32 k of both sides through LDS (k-major, so a thread reads its
+// 4 queries and 4 candidates as one float4 each).  Each tile leaves, per
+// query, the lowest (distance, index) of its candidates in
+// part[tile_c][slot]; tiles past the list's length (read on the device)
+// return at once.
+constexpr int kRT = 64;
+constexpr int kRK = 32;
+
+__global__ __launch_bounds__(256) void nn_refine_tile_kernel(
+    const float *__restrict__ Q, const float *__restrict__ R, int nr, int dim,
+    const int *__restrict__ todo, const int *__restrict__ todo_count, int slots,
+    double2 *__restrict__ part) {
+    __shared__ __attribute__((aligned(16))) float sq[kRK][kRT + 4];
+    __shared__ __attribute__((aligned(16))) float sc[kRK][kRT + 4];
     const int count = *todo_count;
-    for (int e = blockIdx.x; e < count; e += gridDim.x) {
-        const int i = todo[e];
+    const int e0 = blockIdx.x * kRT;
+    if (e0 >= count) return;  // uniform over the block
+    const int c0 = blockIdx.y * kRT;
+    const int tid = threadIdx.x, tq = tid >> 4, tc = tid & 15;
+    // loaders: row tid/4 of each tile, 8 consecutive k at (tid%4)*8
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    const float *qrow = Q + (long)todo[min(e0 + lr, count - 1)] * dim;
+    const float *crow = R + (long)min(c0 + lr, nr - 1) * dim;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int k0 = 0; k0 < dim; k0 += kRK) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + lk + j;
+            sq[lk + j][lr] = k < dim ? qrow[k] : 0.0f;
+            sc[lk + j][lr] = k < dim ? crow[k] : 0.0f;
+        }
         __syncthreads();
-        for (int k = tid; k < dim; k += 256) q[k] = Q[(long)i * dim + k];
+        const int kn = min(kRK, dim - k0);
+        for (int kk = 0; kk < kn; ++kk) {
+            const float4 qv = *reinterpret_cast<const float4 *>(&sq[kk][tq * 4]);
+            const float4 cv = *reinterpret_cast<const float4 *>(&sc[kk][tc * 4]);
+            const double qd[4] = {qv.x, qv.y, qv.z, qv.w};
+            const double cd[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const double d = qd[a] - cd[b];
+                    acc[a][b] = fma(d, d, acc[a][b]);
+                }
+        }
         __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
         double best = __builtin_huge_val();
         int bi = INT_MAX;
-        for (int c = wave; c < nr; c += 4) {
-            const double s = sqdist64(q, R + (long)c * dim, dim, lane);
-            if (s < best) {
-                best = s;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = c0 + tc * 4 + b;
+            if (c < nr && acc[a][b] < best) {
+                best = acc[a][b];
                 bi = c;
             }
         }
-        if (lane == 0) {
-            wbest[wave] = best;
-            widx[wave] = bi;
+        // lowest (distance, index) over the 16 threads of this query group
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            const double ob = __shfl_xor(best, m);
+            const int oi = __shfl_xor(bi, m);
+            if (ob < best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
         }
-        __syncthreads();
-        if (tid == 0) {
-            double b = wbest[0];
-            int w = widx[0];
-            for (int k = 1; k < 4; ++k)
-                if (wbest[k] < b || (wbest[k] == b && widx[k] < w)) {
-                    b = wbest[k];
-                    w = widx[k];
-                }
-            nn[i] = w == INT_MAX ? -1 : w;
+        const int e = e0 + tq * 4 + a;
+        if (tc == 0 && e < count)
+            part[(long)blockIdx.y * slots + e] = make_double2(best, int_to_dbits(bi));
+    }
+}
+
+// Lowest (distance, index) over the candidate tiles for each listed query.
+__global__ void nn_refine_merge_kernel(const double2 *__restrict__ part, int ctiles, int slots,
+                                       const int *__restrict__ todo,
+                                       const int *__restrict__ todo_count, int *__restrict__ nn) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *todo_count) return;
+    double best = __builtin_huge_val();
+    int bi = INT_MAX;
+    for (int t = 0; t < ctiles; ++t) {
+        const double2 v = part[(long)t * slots + e];
+        const int vi = dbits_to_int(v.y);
+        if (v.x < best || (v.x == best && vi < bi)) {
+            best = v.x;
+            bi = vi;
         }
     }
+    nn[todo[e]] = bi == INT_MAX ? -1 : bi;
 }
 
 // Euclidean distance of each query to its neighbour (knnsearch's second
@@ -850,11 +956,15 @@ size_t carved(size_t count, size_t elem) { return (count * elem + 255) & ~size_t
 
 namespace tdm {
 
-size_t match_scratch_bytes(int num_a, int num_b) {
-    const size_t na = num_a, nb = num_b;
+size_t match_scratch_bytes(int num_a, int num_b, int dim) {
+    const size_t na = num_a, nb = num_b, d = dim;
     const size_t tiles_a = (na + kTile - 1) / kTile, tiles_b = (nb + kTile - 1) / kTile;
+    const size_t rt_a = (na + kRT - 1) / kRT, rt_b = (nb + kRT - 1) / kRT;
+    const size_t chunks = (na + kRT - 1) / kRT + (nb + kRT - 1) / kRT;
     return carved(tiles_b * na, 16) + carved(tiles_a * nb, 16) + carved(4, 4) +
-           2 * carved(na, 4) + 2 * carved(nb, 4) + 2 * carved(na, 4) + 2 * carved(nb, 4);
+           2 * carved(na, 4) + 2 * carved(nb, 4) + 2 * carved(na, 4) + 2 * carved(nb, 4) +
+           carved(na * d, 4) + carved(nb * d, 4) + carved(chunks * d, 8) + carved(d, 4) +
+           carved(rt_b * na, 16) + carved(rt_a * nb, 16);
 }
 
 size_t ransac_scratch_bytes(long max_trials) {
@@ -882,10 +992,12 @@ int tdm_match_nn(tdm_ctx *ctx, const float *desc_a, int num_a, const float *desc
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     hipStream_t s = tdm::as_stream(stream);
     const int tiles_a = (num_a + kTile - 1) / kTile, tiles_b = (num_b + kTile - 1) / kTile;
-    TDM_CHECK(tiles_b <= 65535, TDM_ERR_INVALID_ARG, "num_b %d too large", num_b);
+    TDM_CHECK(tiles_b <= 65535 && (num_a + kRT - 1) / kRT <= 65535 &&
+                  (num_b + kRT - 1) / kRT <= 65535,
+              TDM_ERR_INVALID_ARG, "descriptor sets too large (%d, %d)", num_a, num_b);
 
     const size_t na = num_a, nb = num_b;
-    const size_t bytes = tdm::match_scratch_bytes(num_a, num_b);
+    const size_t bytes = tdm::match_scratch_bytes(num_a, num_b, dim);
     void *base = nullptr;
     TDM_TRY(tdm::ctx_scratch(ctx, 2, bytes, &base, s));
     unsigned char *cur = static_cast<unsigned char *>(base);
@@ -896,17 +1008,39 @@ int tdm_match_nn(tdm_ctx *ctx, const float *desc_a, int num_a, const float *desc
     float *sq_b = carve<float>(cur, nb), *len_b = carve<float>(cur, nb);
     int *todo_a = carve<int>(cur, na), *todo_b = carve<int>(cur, nb);
     int *own_ab = carve<int>(cur, na), *own_ba = carve<int>(cur, nb);
+    float *cen_a = carve<float>(cur, na * dim), *cen_b = carve<float>(cur, nb * dim);
+    const int chunks_a = (num_a + kRT - 1) / kRT, chunks_b = (num_b + kRT - 1) / kRT;
+    double *colsum = carve<double>(cur, (size_t)(chunks_a + chunks_b) * dim);
+    float *mean = carve<float>(cur, dim);
+    double2 *rpart_a = carve<double2>(cur, (size_t)chunks_b * na);
+    double2 *rpart_b = carve<double2>(cur, (size_t)chunks_a * nb);
     int *out_ab = nn_ab ? nn_ab : own_ab, *out_ba = nn_ba ? nn_ba : own_ba;
     unsigned *maxlen = reinterpret_cast<unsigned *>(small);
 
     clear_ints_kernel<<<1, 64, 0, s>>>(small, 4);
     TDM_LAUNCHED();
-    norms_kernel<<<tdm::blocks_for(na, 4), 256, 0, s>>>(desc_a, num_a, dim, sq_a, len_a, maxlen);
+    // centring shift: the mean of both sets
+    const unsigned cb = tdm::blocks_for(dim, 256);
+    colsum_kernel<<<dim3(cb, chunks_a), 256, 0, s>>>(desc_a, num_a, dim, kRT, colsum);
     TDM_LAUNCHED();
-    norms_kernel<<<tdm::blocks_for(nb, 4), 256, 0, s>>>(desc_b, num_b, dim, sq_b, len_b,
+    colsum_kernel<<<dim3(cb, chunks_b), 256, 0, s>>>(desc_b, num_b, dim, kRT,
+                                                     colsum + (size_t)chunks_a * dim);
+    TDM_LAUNCHED();
+    mean_kernel<<<cb, 256, 0, s>>>(colsum, chunks_a + chunks_b, dim, (double)na + (double)nb,
+                                   mean);
+    TDM_LAUNCHED();
+    const unsigned ctb = (unsigned)std::min<size_t>(tdm::blocks_for(na * dim, 256), 4096);
+    center_kernel<<<ctb, 256, 0, s>>>(desc_a, (long)(na * dim), dim, mean, cen_a);
+    TDM_LAUNCHED();
+    const unsigned ctb2 = (unsigned)std::min<size_t>(tdm::blocks_for(nb * dim, 256), 4096);
+    center_kernel<<<ctb2, 256, 0, s>>>(desc_b, (long)(nb * dim), dim, mean, cen_b);
+    TDM_LAUNCHED();
+    norms_kernel<<<tdm::blocks_for(na, 4), 256, 0, s>>>(cen_a, num_a, dim, sq_a, len_a, maxlen);
+    TDM_LAUNCHED();
+    norms_kernel<<<tdm::blocks_for(nb, 4), 256, 0, s>>>(cen_b, num_b, dim, sq_b, len_b,
                                                          maxlen + 1);
     TDM_LAUNCHED();
-    dist_top2_kernel<<<dim3(tiles_a, tiles_b), 256, 0, s>>>(desc_a, num_a, desc_b, num_b, dim,
+    dist_top2_kernel<<<dim3(tiles_a, tiles_b), 256, 0, s>>>(cen_a, num_a, cen_b, num_b, dim,
                                                             sq_a, sq_b, part_a, part_b);
     TDM_LAUNCHED();
     nn_select_kernel<<<tdm::blocks_for(na, 256), 256, 0, s>>>(part_a, tiles_b, num_a, dim, len_a,
@@ -916,11 +1050,18 @@ int tdm_match_nn(tdm_ctx *ctx, const float *desc_a, int num_a, const float *desc
     nn_select_kernel<<<tdm::blocks_for(nb, 256), 256, 0, s>>>(part_b, tiles_a, num_b, dim, len_b,
                                                               maxlen, out_ba, todo_b, small + 3);
     TDM_LAUNCHED();
-    nn_refine_kernel<<<std::min(num_a, 1024), 256, 0, s>>>(desc_a, desc_b, num_b, dim, todo_a,
-                                                           small + 2, out_ab);
+    // exact fp64 search of the uncertified queries, on the original vectors
+    nn_refine_tile_kernel<<<dim3(chunks_a, chunks_b), 256, 0, s>>>(
+        desc_a, desc_b, num_b, dim, todo_a, small + 2, num_a, rpart_a);
     TDM_LAUNCHED();
-    nn_refine_kernel<<<std::min(num_b, 1024), 256, 0, s>>>(desc_b, desc_a, num_a, dim, todo_b,
-                                                           small + 3, out_ba);
+    nn_refine_merge_kernel<<<tdm::blocks_for(na, 256), 256, 0, s>>>(rpart_a, chunks_b, num_a,
+                                                                    todo_a, small + 2, out_ab);
+    TDM_LAUNCHED();
+    nn_refine_tile_kernel<<<dim3(chunks_b, chunks_a), 256, 0, s>>>(
+        desc_b, desc_a, num_a, dim, todo_b, small + 3, num_b, rpart_b);
+    TDM_LAUNCHED();
+    nn_refine_merge_kernel<<<tdm::blocks_for(nb, 256), 256, 0, s>>>(rpart_b, chunks_a, num_b,
+                                                                    todo_b, small + 3, out_ba);
     TDM_LAUNCHED();
     if (dist_ab) {
         nn_dist_kernel<<<tdm::blocks_for(na, 4), 256, 0, s>>>(desc_a, desc_b, num_a, dim, out_ab,

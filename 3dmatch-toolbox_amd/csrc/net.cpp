@@ -39,6 +39,7 @@ struct Layer {
     bool dirty = false;      // host weights newer than the device copy
     void *w_dev = nullptr;   // packed [N][taps][C] (conv1: [N][KP])
     float *b_dev = nullptr;
+    int variant = -1;        // conv tile variant (-1: default; see tdm_net_autotune)
     int final_d() const { return pool ? pooled_d : out_d; }
     size_t out_elems() const { return (size_t)cout * final_d() * final_d() * final_d(); }
 };
@@ -353,7 +354,7 @@ int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *
         const int splits = choose_splits(net, l, nq);
         void *dst = (l.pool && !l.pool_fused) ? net.act[1 - cur] : out;
         TDM_TRY(tdm::launch_conv(net.dt, out_f32, d, n, net.act[cur], l.w_dev, l.b_dev, dst,
-                                 splits, net.partial, s));
+                                 splits, net.partial, l.variant, s));
         cur = 1 - cur;
         if (l.pool && !l.pool_fused) {
             TDM_CHECK(!to_final, TDM_ERR_UNSUPPORTED, "pooling after the last layer");
@@ -636,6 +637,76 @@ int tdm_net_forward_grids(tdm_net *net, const tdm_grid_ref *grids, const int *kp
         src.P = net->patch;
         TDM_TRY(run_pass(*net, src, n, (int)net->layers.size() - 1, feat + (long)start * D, nullptr, s));
     }
+    return TDM_OK;
+}
+
+int tdm_net_autotune(tdm_net *net, int num, void *stream) {
+    tdm::clear_error();
+    TDM_TRY(check_ready(net));
+    TDM_CHECK(num >= 1, TDM_ERR_INVALID_ARG, "batch %d", num);
+    const int n = std::min(num, net->max_batch);
+    hipStream_t s = tdm::as_stream(stream);
+    // finite inputs for the timing runs (results are discarded)
+    TDM_HIP(hipMemsetAsync(net->act[0], 0, net->act_bytes, s));
+    hipEvent_t e0, e1;
+    TDM_HIP(hipEventCreate(&e0));
+    TDM_HIP(hipEventCreate(&e1));
+    int st = TDM_OK;
+    for (size_t li = 1; li < net->layers.size() && st == TDM_OK; ++li) {
+        Layer &l = net->layers[li];
+        tdm::ConvDesc d;
+        d.C = l.cin;
+        d.N = l.cout;
+        d.D = l.in_d;
+        d.OD = l.out_d;
+        d.ksize = l.k;
+        d.relu = l.relu;
+        d.pool = l.pool_fused;
+        if (!tdm::conv_uses_lds(net->dt, d)) continue;
+        const size_t nq = (size_t)tdm::conv_reduction_steps(net->dt, d);
+        const int splits = choose_splits(*net, l, nq);
+        const bool out_f32 = net->dt == tdm::DType::F32 || li + 1 == net->layers.size();
+        float best = 3.4e38f;
+        int best_v = tdm::conv_default_variant(d);
+        for (int v = 0; v < tdm::kConvVariants && st == TDM_OK; ++v) {
+            if (!tdm::conv_variant_valid(v, d)) continue;
+            auto run = [&]() {
+                return tdm::launch_conv(net->dt, out_f32, d, n, net->act[0], l.w_dev, l.b_dev,
+                                        net->act[1], splits, net->partial, v, s);
+            };
+            if ((st = run()) != TDM_OK) break;  // warm-up
+            if (hipEventRecord(e0, s) != hipSuccess) st = TDM_ERR_HIP;
+            for (int r = 0; r < 3 && st == TDM_OK; ++r) st = run();
+            if (st == TDM_OK && hipEventRecord(e1, s) != hipSuccess) st = TDM_ERR_HIP;
+            float ms = 0.0f;
+            if (st == TDM_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                st = TDM_ERR_HIP;
+            if (st == TDM_OK && ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        l.variant = best_v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st == TDM_ERR_HIP) return tdm::fail(st, "autotune timing failed");
+    return st;
+}
+
+int tdm_net_layer_variant(const tdm_net *net, int index) {
+    if (!net || index < 0 || index >= (int)net->layers.size()) return -2;
+    return net->layers[index].variant;
+}
+
+int tdm_net_set_layer_variant(tdm_net *net, int index, int variant) {
+    tdm::clear_error();
+    TDM_CHECK(net && index >= 0 && index < (int)net->layers.size(), TDM_ERR_INVALID_ARG,
+              "bad layer index");
+    TDM_CHECK(variant >= -1 && variant < tdm::kConvVariants, TDM_ERR_INVALID_ARG,
+              "bad variant %d", variant);
+    net->layers[index].variant = variant;
     return TDM_OK;
 }
 

@@ -12,8 +12,8 @@ namespace tdm {
 int grid_from_box(const float box[6], float voxel_size, int padding, int dims[3],
                   float origin[3]);
 
-// Scratch (context slot 2) used by tdm_match_nn for num_a x num_b.
-size_t match_scratch_bytes(int num_a, int num_b);
+// Scratch (context slot 2) used by tdm_match_nn for num_a x num_b of width dim.
+size_t match_scratch_bytes(int num_a, int num_b, int dim);
 // Scratch (context slot 3) used by tdm_ransac_rt for max_trials.
 size_t ransac_scratch_bytes(long max_trials);
 

@@ -532,7 +532,7 @@ int server_init(tdm_server *s) {
         TDM_HIP(hipHostMalloc((void **)&sl.res, sizeof(DevResult), hipHostMallocDefault));
         // matching and RANSAC scratch up front, so requests do not allocate
         void *p = nullptr;
-        TDM_TRY(tdm::ctx_scratch(sl.ctx, 2, tdm::match_scratch_bytes(c.keypoints, c.keypoints),
+        TDM_TRY(tdm::ctx_scratch(sl.ctx, 2, tdm::match_scratch_bytes(c.keypoints, c.keypoints, s->dim),
                                  &p, sl.stream));
         TDM_TRY(tdm::ctx_scratch(sl.ctx, 3, tdm::ransac_scratch_bytes(c.ransac_max_trials), &p,
                                  sl.stream));

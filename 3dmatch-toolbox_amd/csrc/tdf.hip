@@ -17,6 +17,7 @@
 // table is non-increasing in d^2.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -171,24 +172,30 @@ __global__ void pack_bits_kernel(const float *__restrict__ occ, int dx, long row
 
 // ---- TDF tile kernel --------------------------------------------------------------
 
-// Truncated squared-distance transform of one 64x16x16 tile.
+// LDS of one tile: halo bit words (dead after pass x, so pass y's output
+// reuses them -- keeps two workgroups per CU at R = 5), pass-x output, table.
 template <int R>
-__global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *__restrict__ bits,
-                                                               int dx, int dy, int dz, Lut lut,
-                                                               float *__restrict__ tdf) {
+struct TileLds {
+    static constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
+    static constexpr int kSwBytes = HZ * HY * 4 * 4, kS2Bytes = HZ * kTY * kTX;
+    unsigned char a[kSwBytes > kS2Bytes ? kSwBytes : kS2Bytes] __attribute__((aligned(16)));
+    unsigned char s1[HZ * HY * kTX];
+    float lut[kLutSize];
+};
+
+// Truncated squared-distance transform of the 64x16x16 tile at (X0, Y0, Z0).
+template <int R>
+__device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits, int dx, int dy,
+                                int dz, const Lut &lut, float *__restrict__ tdf, int X0, int Y0,
+                                int Z0) {
     constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
-    // The halo bit words (x in [X0-32, X0+96)) are dead after pass x, so pass
-    // y's output reuses their LDS (keeps two workgroups per CU at R = 5).
-    constexpr int kSwBytes = HZ * HY * 4 * 4, kS2Bytes = HZ * kTY * kTX;
-    __shared__ __attribute__((aligned(16))) unsigned char shared_a[kSwBytes > kS2Bytes ? kSwBytes : kS2Bytes];
-    unsigned *sw = reinterpret_cast<unsigned *>(shared_a);
-    unsigned char *s2 = shared_a;
-    __shared__ unsigned char s1[HZ * HY * kTX];  // pass x, all halo rows
-    __shared__ float slut[kLutSize];
+    unsigned *sw = reinterpret_cast<unsigned *>(L.a);
+    unsigned char *s2 = L.a;
+    unsigned char *s1 = L.s1;
+    float *slut = L.lut;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int X0 = blockIdx.x * kTX, Y0 = blockIdx.y * kTY, Z0 = blockIdx.z * kTZ;
     const long rw = row_words(dx);
     const long wbase = X0 >> 5;  // first word of the tile in a row
 
@@ -270,6 +277,75 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *_
         }
         tdf[((long)gz * dy + gy) * dx + x] = val;
     }
+}
+
+template <int R>
+__global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *__restrict__ bits,
+                                                               int dx, int dy, int dz, Lut lut,
+                                                               float *__restrict__ tdf) {
+    __shared__ TileLds<R> L;
+    tdf_tile<R>(L, bits, dx, dy, dz, lut, tdf, blockIdx.x * kTX, blockIdx.y * kTY,
+                blockIdx.z * kTZ);
+}
+
+// ---- many clouds in one launch per step ---------------------------------------
+
+// One cloud of a batch (device table, filled by the plan on the host).
+struct CloudDev {
+    const float *pts;
+    float *tdf;
+    long pt_off;    // first point of this cloud in the batch's point numbering
+    long word_off;  // first bit word of this cloud's grid in the batch bit buffer
+    long tile_off;  // first tile of this cloud in the batch's tile numbering
+    int n;
+    int dims[3];
+    float origin[3];
+    int tiles[3];
+};
+
+// Last cloud c with key(c) <= v (keys ascending, c in [0, num)).
+template <typename Key>
+__device__ inline int find_cloud(const CloudDev *__restrict__ clouds, int num, long v, Key key) {
+    int lo = 0, hi = num - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(clouds[mid]) <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void scatter_bits_batch_kernel(const CloudDev *__restrict__ clouds, int num,
+                                          long total, float vs, unsigned *__restrict__ bits) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const int c = find_cloud(clouds, num, i, [](const CloudDev &d) { return d.pt_off; });
+        const CloudDev &cd = clouds[c];
+        const long j = i - cd.pt_off;
+        if (j >= cd.n) continue;  // (an empty cloud shares its offset with the next)
+        int gx, gy, gz;
+        if (!point_voxel(cd.pts, j, cd.dims[0], cd.dims[1], cd.dims[2], cd.origin[0],
+                         cd.origin[1], cd.origin[2], vs, gx, gy, gz))
+            continue;
+        const long w =
+            cd.word_off + ((long)gz * cd.dims[1] + gy) * row_words(cd.dims[0]) + (gx >> 5);
+        atomicOr(&bits[w], 1u << (gx & 31));
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kTdfThreads) void tdf_tile_batch_kernel(
+    const CloudDev *__restrict__ clouds, int num, const unsigned *__restrict__ bits, Lut lut) {
+    __shared__ TileLds<R> L;
+    const long t = blockIdx.x;
+    const int c = find_cloud(clouds, num, t, [](const CloudDev &d) { return d.tile_off; });
+    const CloudDev &cd = clouds[c];
+    const long lt = t - cd.tile_off;
+    const int tx = (int)(lt % cd.tiles[0]);
+    const int ty = (int)((lt / cd.tiles[0]) % cd.tiles[1]);
+    const int tz = (int)(lt / ((long)cd.tiles[0] * cd.tiles[1]));
+    tdf_tile<R>(L, bits + cd.word_off, cd.dims[0], cd.dims[1], cd.dims[2], lut, cd.tdf, tx * kTX,
+                ty * kTY, tz * kTZ);
 }
 
 // float32 decode of core/demo.cu:32-34.
@@ -488,7 +564,8 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
                         const float origin[3], float voxel_size, float trunc_margin, int decode,
                         float *tdf, void *stream) {
     tdm::clear_error();
-    TDM_CHECK(pts && dims && origin && tdf, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK((pts || num_pts == 0) && dims && origin && tdf, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
     TDM_CHECK(num_pts >= 0, TDM_ERR_INVALID_ARG, "negative point count");
     TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
               "bad decode mode %d", decode);
@@ -538,6 +615,120 @@ int tdm_extract_patches(tdm_ctx *ctx, const float *tdf, int dim_x, int dim_y, in
     extract_patches_kernel<<<blocks, 256, 0, tdm::as_stream(stream)>>>(
         tdf, dim_x, dim_y, dim_z, kp_grid, num, patch_size, patches);
     TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+struct tdm_tdf_plan {
+    int device = 0;
+    int num = 0;
+    int r = 0;
+    int decode = TDM_DECODE_EXACT;
+    float vs = 0.0f;
+    long total_pts = 0, total_tiles = 0;
+    size_t total_words = 0;
+    CloudDev *table = nullptr;   // device
+    unsigned *bits = nullptr;    // device, all clouds' bit grids
+    std::vector<CloudDev> host;  // the same table on the host
+};
+
+int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, float voxel_size,
+                        float trunc_margin, int decode, tdm_tdf_plan **out) {
+    tdm::clear_error();
+    TDM_CHECK(clouds && out && num >= 1, TDM_ERR_INVALID_ARG, "bad arguments");
+    TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
+              "bad decode mode %d", decode);
+    int r = 0;
+    TDM_TRY(search_radius(voxel_size, trunc_margin, &r));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    auto plan = std::make_unique<tdm_tdf_plan>();
+    plan->device = ctx->device;
+    plan->num = num;
+    plan->r = r;
+    plan->decode = decode;
+    plan->vs = voxel_size;
+    plan->host.resize(num);
+    long pts = 0, tiles = 0;
+    size_t words = 0;
+    for (int c = 0; c < num; ++c) {
+        const tdm_cloud_ref &in = clouds[c];
+        TDM_CHECK(in.tdf && in.num_pts >= 0 && (in.pts || in.num_pts == 0), TDM_ERR_INVALID_ARG,
+                  "cloud %d: bad pointers", c);
+        TDM_TRY(check_dims(in.dim_x, in.dim_y, in.dim_z));
+        CloudDev &d = plan->host[c];
+        d.pts = in.pts;
+        d.tdf = in.tdf;
+        d.n = in.num_pts;
+        d.dims[0] = in.dim_x;
+        d.dims[1] = in.dim_y;
+        d.dims[2] = in.dim_z;
+        d.origin[0] = in.origin_x;
+        d.origin[1] = in.origin_y;
+        d.origin[2] = in.origin_z;
+        d.tiles[0] = (in.dim_x + kTX - 1) / kTX;
+        d.tiles[1] = (in.dim_y + kTY - 1) / kTY;
+        d.tiles[2] = (in.dim_z + kTZ - 1) / kTZ;
+        d.pt_off = pts;
+        d.word_off = (long)words;
+        d.tile_off = tiles;
+        pts += in.num_pts;
+        tiles += (long)d.tiles[0] * d.tiles[1] * d.tiles[2];
+        words += (size_t)row_words(in.dim_x) * in.dim_y * in.dim_z;
+    }
+    TDM_CHECK(tiles < (1L << 31), TDM_ERR_INVALID_ARG, "batch too large");
+    plan->total_pts = pts;
+    plan->total_tiles = tiles;
+    plan->total_words = words;
+    TDM_HIP(hipMalloc((void **)&plan->table, sizeof(CloudDev) * num));
+    TDM_HIP(hipMalloc((void **)&plan->bits, words * sizeof(unsigned)));
+    TDM_HIP(hipMemcpy(plan->table, plan->host.data(), sizeof(CloudDev) * num,
+                      hipMemcpyHostToDevice));
+    *out = plan.release();
+    return TDM_OK;
+}
+
+int tdm_tdf_plan_run(tdm_tdf_plan *plan, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(plan, TDM_ERR_INVALID_ARG, "NULL plan");
+    TDM_HIP(hipSetDevice(plan->device));
+    hipStream_t s = tdm::as_stream(stream);
+    const unsigned fb = (unsigned)std::min<size_t>(tdm::blocks_for(plan->total_words, 256), 16384);
+    fill_u32_kernel<<<fb, 256, 0, s>>>(plan->bits, plan->total_words, 0u);
+    TDM_LAUNCHED();
+    if (plan->total_pts > 0) {
+        const unsigned sb = (unsigned)std::min<long>((plan->total_pts + 255) / 256, 65536);
+        scatter_bits_batch_kernel<<<sb, 256, 0, s>>>(plan->table, plan->num, plan->total_pts,
+                                                      plan->vs, plan->bits);
+        TDM_LAUNCHED();
+    }
+    const Lut lut = make_lut(plan->r);
+    const unsigned tb = (unsigned)plan->total_tiles;
+    switch (plan->r) {
+#define TDM_TB(R) \
+    case R: tdf_tile_batch_kernel<R><<<tb, kTdfThreads, 0, s>>>(plan->table, plan->num, plan->bits, lut); break;
+        TDM_TB(1) TDM_TB(2) TDM_TB(3) TDM_TB(4) TDM_TB(5) TDM_TB(6) TDM_TB(7) TDM_TB(8)
+#undef TDM_TB
+    default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", plan->r);
+    }
+    TDM_LAUNCHED();
+    if (plan->decode == TDM_DECODE_REFERENCE)
+        for (const CloudDev &d : plan->host) {
+            const long n = (long)d.dims[0] * d.dims[1] * d.dims[2];
+            const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
+            tdf_refdecode_kernel<<<blocks, 256, 0, s>>>(plan->bits + d.word_off, d.dims[0],
+                                                         d.dims[1], d.dims[2], plan->r, lut, d.tdf);
+            TDM_LAUNCHED();
+        }
+    return TDM_OK;
+}
+
+int tdm_tdf_plan_destroy(tdm_tdf_plan *plan) {
+    tdm::clear_error();
+    if (!plan) return TDM_OK;
+    (void)hipSetDevice(plan->device);
+    (void)hipDeviceSynchronize();
+    if (plan->table) (void)hipFree(plan->table);
+    if (plan->bits) (void)hipFree(plan->bits);
+    delete plan;
     return TDM_OK;
 }
 

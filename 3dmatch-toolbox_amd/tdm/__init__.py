@@ -169,6 +169,44 @@ def tdf_from_points(pts, dims, origin, voxel_size, trunc, tdf, decode=DECODE_EXA
                                      _p(tdf), _stream(stream)))
 
 
+class _CloudRef(ctypes.Structure):
+    _fields_ = [("pts", c_void_p), ("tdf", c_void_p), ("num_pts", c_int), ("dim_x", c_int),
+                ("dim_y", c_int), ("dim_z", c_int), ("origin_x", c_float), ("origin_y", c_float),
+                ("origin_z", c_float), ("reserved", c_int)]
+
+
+class TdfPlan:
+    """tdm_tdf_plan: the TDF of many clouds in three launches per run
+    (capturable in a hipGraph).  clouds: list of (pts tensor [n,3], tdf tensor,
+    dims, origin); the tensors must outlive the plan."""
+
+    def __init__(self, clouds, voxel_size=0.01, trunc=0.05, decode=DECODE_EXACT):
+        arr = (_CloudRef * len(clouds))()
+        for i, (pts, tdf, dims, origin) in enumerate(clouds):
+            o = np.asarray(origin, np.float32)
+            arr[i] = _CloudRef(pts.data_ptr() if pts.numel() else None, tdf.data_ptr(),
+                               int(pts.shape[0]), int(dims[0]), int(dims[1]), int(dims[2]),
+                               float(o[0]), float(o[1]), float(o[2]), 0)
+        self._keep = clouds
+        self._h = c_void_p()
+        _check(lib().tdm_tdf_plan_create(None, arr, c_int(len(clouds)), _f(voxel_size), _f(trunc),
+                                         c_int(decode), ctypes.byref(self._h)))
+
+    def run(self, stream=None) -> None:
+        _check(lib().tdm_tdf_plan_run(self._h, _stream(stream)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().tdm_tdf_plan_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def keypoints_to_grid(kp: np.ndarray, origin, voxel_size) -> np.ndarray:
     kp = np.ascontiguousarray(kp, dtype=np.float32).reshape(-1, 3)
     out = np.zeros(kp.shape, dtype=np.int32)
@@ -327,6 +365,7 @@ class Net:
         _check(lib().tdm_net_create(None, json_path.encode(), c_int(precision), c_int(max_batch),
                                     ctypes.byref(self._h)))
         self.precision = precision
+        self.max_batch = max_batch if max_batch > 0 else 256
 
     def close(self) -> None:
         if self._h:
@@ -383,6 +422,18 @@ class Net:
         _check(lib().tdm_net_forward_grid(self._h, _p(tdf), c_int(dims[0]), c_int(dims[1]),
                                           c_int(dims[2]), _p(kp_grid), c_int(kp_grid.shape[0]),
                                           _p(feat), _stream(stream)))
+
+    def autotune(self, num: int = 0, stream=None) -> list[int]:
+        """Times each convolution's tile variants and keeps the fastest;
+        returns the chosen variant per layer."""
+        _check(lib().tdm_net_autotune(self._h, c_int(num or self.max_batch), _stream(stream)))
+        return [self.layer_variant(i) for i in range(len(self.layer_names))]
+
+    def layer_variant(self, index: int) -> int:
+        return int(lib().tdm_net_layer_variant(self._h, c_int(index)))
+
+    def set_layer_variant(self, index: int, variant: int) -> None:
+        _check(lib().tdm_net_set_layer_variant(self._h, c_int(index), c_int(variant)))
 
     def forward_grids(self, table, kp_grid, grid_index, feat, stream=None) -> None:
         """table: device tensor from grid_table(); kp_grid int32 [n, 3] and

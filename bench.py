@@ -12,8 +12,13 @@ fused single pass and per-frame launches) in voxel-updates/s with its HBM
 roofline, the bf16 descriptor path, per-layer times, and the CPU baseline
 (the oracle's C loop nests on a bounded sample, scaled).
 
+Also: the fp16 path (the reference's default half storage), a precision
+study (rel-L2 vs FP64 per layer, exact mutual-NN agreement), matching and
+RANSAC kernel times, and the serving form (p50/p99 registration latency,
+Poisson arrival sweep) of the registration server.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-                       [--no-cpu-baseline] [--no-extras]
+                       [--no-cpu-baseline] [--no-extras] [--no-serving]
 """
 from __future__ import annotations
 
@@ -29,7 +34,7 @@ sys.path.insert(0, os.path.join(REPO, "3dmatch-toolbox_amd"))
 
 METRIC = "3DMatch descriptors/s + TSDF voxel-updates/s on 1 GPU; p50/p99 reg. latency"
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0}
 
 
 def log(msg: str) -> None:
@@ -60,6 +65,8 @@ class DescriptorPipeline:
         self.keypts = keypts
         self.net = tdm.Net(js, precision, max_batch=batch)
         self.net.load_weights(wt)
+        # tile autotuner: bit-identical results, fastest tile per layer
+        self.variants = self.net.autotune(batch)
         self.vs = 0.01
         self.trunc = 0.05
         dims, _ = tdm.grid_params(self.pts, self.vs, 15)
@@ -80,6 +87,17 @@ class DescriptorPipeline:
         self.kp_host.copy_(torch.from_numpy(kg))
         self.kp_dev.copy_(self.kp_host, non_blocking=True)
         self.net.forward_grid(self.tdf, dims, self.kp_dev, self.feat)
+
+
+def layer_tflops(net):
+    from tdm import netdef
+
+    out = {}
+    for i, (name, cin, cout, k, _d, od, _p) in enumerate(netdef.layer_shapes()):
+        ms, passes, patches = net.layer_stats(i)
+        flops = 2 * od ** 3 * cout * cin * k ** 3
+        out[name] = flops * patches / (ms / 1e3) / 1e12 if ms > 0 else None
+    return out
 
 
 def time_steps(torch, fn, steps, warmup):
@@ -152,11 +170,14 @@ def tsdf_bench(torch, reps=3):
 
 
 def mutual_nn(torch, a, b):
-    """Nearest neighbours of a's rows in b and mutual pairs, by L2 distance
-    (the matching of core/demo.m:49-55)."""
-    d = torch.cdist(a, b)
-    ab = d.argmin(dim=1)
-    ba = d.argmin(dim=0)
+    """Exact nearest neighbours of a's rows in b and the mutual mask, with the
+    library's matcher (the knnsearch pair of core/demo.m:49-55)."""
+    import tdm
+
+    ab = torch.empty(len(a), dtype=torch.int32, device=a.device)
+    ba = torch.empty(len(b), dtype=torch.int32, device=a.device)
+    tdm.match_nn(a.contiguous(), b.contiguous(), ab, ba)
+    ab, ba = ab.long(), ba.long()
     idx = torch.arange(len(a), device=a.device)
     mutual = ba[ab] == idx
     return ab, mutual
@@ -237,6 +258,169 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
     }
 
 
+def _rot_err_deg(Ra, Rb):
+    import numpy as np
+
+    c = (np.trace(Ra.T @ Rb) - 1.0) / 2.0
+    return float(np.degrees(np.arccos(np.clip(c, -1.0, 1.0))))
+
+
+def registration_kernels(torch, js, wt, n_kp=5000, trials=50000):
+    """Matching and RANSAC on one request's real descriptors: mutual NN of
+    two 5000 x 512 sets (one 25.6 GFLOP fp32 distance GEMM on MFMA plus the
+    top-2 epilogue) and RANSAC with 50k hypotheses, timed with HIP events."""
+    import numpy as np
+
+    import tdm
+    from tdm import synth
+
+    dev = torch.device("cuda:0")
+    net = tdm.Net(js, tdm.PREC_FP16, max_batch=1024)
+    net.load_weights(wt)
+    p1, p2, R, t = synth.fragment_pair(7, 200_000, 200_000)
+    feats, kps = [], []
+    rng = np.random.default_rng(8)
+    for pts in (p1, p2):
+        dp = torch.from_numpy(pts).to(dev)
+        dims, origin = tdm.grid_params(dp, 0.01, 15)
+        tdf = torch.empty(dims[2], dims[1], dims[0], device=dev)
+        tdm.tdf_from_points(dp, dims, origin, np.float32(0.01), np.float32(0.05), tdf)
+        kidx = rng.integers(0, len(pts), n_kp)
+        kg = tdm.keypoints_to_grid(pts[kidx], origin, np.float32(0.01))
+        f = torch.empty(n_kp, 512, device=dev)
+        net.forward_grid(tdf, dims, torch.from_numpy(kg).to(dev), f)
+        feats.append(f)
+        kps.append(torch.from_numpy(np.ascontiguousarray(pts[kidx])).to(dev))
+    net.close()
+    ab = torch.empty(n_kp, dtype=torch.int32, device=dev)
+    ba = torch.empty(n_kp, dtype=torch.int32, device=dev)
+    refined = torch.zeros(2, dtype=torch.int32, device=dev)
+    pairs = torch.empty((n_kp, 2), dtype=torch.int32, device=dev)
+    npairs = torch.zeros(1, dtype=torch.int32, device=dev)
+    rt = torch.zeros(12, dtype=torch.float64, device=dev)
+    inl = torch.empty(n_kp, dtype=torch.int32, device=dev)
+    stats = torch.zeros(4, dtype=torch.int32, device=dev)
+
+    def match():
+        tdm.match_nn(feats[0], feats[1], ab, ba, refined=refined)
+        tdm.mutual_pairs(ab, ba, pairs, npairs)
+
+    def ransac():
+        tdm.ransac_rt(kps[0], kps[1], rt, inl, stats, pairs=pairs, num_pairs=npairs,
+                      max_pairs=n_kp, threshold=0.05, max_trials=trials, seed=1)
+
+    out = {}
+    for name, fn in (("match_ms", match), ("ransac_ms", ransac)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps
+    flops = 2.0 * n_kp * n_kp * 512
+    st = stats.cpu().numpy()
+    out.update({
+        "keypoints": n_kp, "dim": 512,
+        "distance_gemm_tflops": flops / (out["match_ms"] / 1e3) / 1e12,
+        "refined_queries": refined.cpu().numpy().tolist(),
+        "mutual_pairs": int(npairs.item()),
+        "ransac_trials_cap": trials, "ransac_trials_run": int(st[2]),
+        "ransac_inliers": int(st[0]),
+    })
+    return out
+
+
+def serving_bench(torch, js, wt, rates=(2.0, 4.0, 8.0), per_rate=16, unloaded=8):
+    """Serving form: fragment-pair registration requests (points U[20k, 500k],
+    50-80 % overlap, rigid motion <= 60 deg / 1 m, 5000 keypoints each, RANSAC
+    50k trials at 5 cm) through the continuous-batching server with an FP16
+    net.  Unloaded latency: one request at a time; then Poisson arrivals at
+    each rate.  Errors are against the generating motion (the weights are
+    random He-normal, not trained, so recovery is not expected to succeed)."""
+    import threading
+
+    import numpy as np
+
+    import tdm
+    from tdm import synth
+
+    net = tdm.Net(js, tdm.PREC_FP16, max_batch=1024)
+    net.load_weights(wt)
+    n_req = unloaded + per_rate * len(rates)
+    reqs = [synth.fragment_pair(70 + k) for k in range(n_req)]
+    srv = tdm.Server(net, memory_budget=24 << 30, max_inflight=8, keypoints=5000,
+                     max_pass_keypoints=20000, ransac_max_trials=50000)
+
+    def summarize(results, wall):
+        lat = np.array([r["latency_ms"] for r in results])
+        return {"requests": len(results), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                "requests_per_s": len(results) / wall,
+                "mean_queue_ms": float(np.mean([r["queue_ms"] for r in results]))}
+
+    # warm-up (first-use allocations)
+    p1, p2, _, _ = reqs[0]
+    srv.wait(srv.submit(p1, p2, seed=1))
+    out = {}
+    res, errs = [], []
+    t0 = time.perf_counter()
+    for k in range(unloaded):
+        p1, p2, R, t = reqs[k]
+        r = srv.wait(srv.submit(p1, p2, seed=100 + k))
+        res.append(r)
+        errs.append((_rot_err_deg(r["rt"][:, :3], R), float(np.linalg.norm(r["rt"][:, 3] - t))))
+    out["unloaded"] = summarize(res, time.perf_counter() - t0)
+    sweep = []
+    rng = np.random.default_rng(9)
+    base = unloaded
+    for rate in rates:
+        batch = reqs[base:base + per_rate]
+        base += per_rate
+        gaps = rng.exponential(1.0 / rate, len(batch))
+        tickets = [None] * len(batch)
+
+        def submitter():
+            start = time.perf_counter()
+            at = 0.0
+            for i, (p1, p2, _, _) in enumerate(batch):
+                at += gaps[i]
+                delay = start + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(p1, p2, seed=1000 + i)
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        results = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        s = summarize(results, wall)
+        s["arrival_rate"] = rate
+        s["mean_pass_keypoints"] = float(np.mean([r["pass_keypoints"] for r in results]))
+        for r, (_, _, R, t) in zip(results, batch):
+            errs.append((_rot_err_deg(r["rt"][:, :3], R), float(np.linalg.norm(r["rt"][:, 3] - t))))
+        sweep.append(s)
+        log(f"serving rate {rate}/s: p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms")
+    out["poisson"] = sweep
+    st = srv.stats()
+    srv.close()
+    net.close()
+    e = np.array(errs)
+    out["server_stats"] = st
+    out["rotation_error_deg_median"] = float(np.median(e[:, 0]))
+    out["translation_error_m_median"] = float(np.median(e[:, 1]))
+    out["recovered_fraction"] = float(np.mean((e[:, 0] < 5.0) & (e[:, 1] < 0.1)))
+    out["config"] = ("fragment pairs of the synthetic scene, points U[20k,500k], overlap "
+                     "U[0.5,0.8], rotation <= 60 deg, shift <= 1 m, 5000 keypoints per "
+                     "fragment, FP16 net, RANSAC 50k trials / 0.05 m, 8 slots, 24 GiB budget")
+    return out
+
+
 def cpu_baseline(cloud, keypts, weights, n_kp, conv_patches=32):
     """Oracle (C loop nests, OpenMP) on a bounded sample of the descriptor
     workload: TDF of 1/8 of the grid's voxels and `conv_patches` patches
@@ -290,6 +474,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving sweep (e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (reference for HBM counters)")
     args = ap.parse_args()
@@ -362,6 +548,7 @@ def main() -> int:
                          "fragment, 0.01 m voxels, truncation 5 voxels, 5000 keypoints, 30^3 "
                          "patches, He-normal weights"),
             "points": len(cloud), "keypoints": n_kp, "grid": pipe.dims, "batch": args.batch,
+            "tile_variants": pipe.variants,
         },
         "roofline": {
             "kernel": "conv2: fp32 MFMA implicit GEMM 64->64 ch, 28^3->26^3, fused bias+ReLU+2^3 max pool",
@@ -394,6 +581,8 @@ def main() -> int:
                 "speedup_vs_fp32": (n_kp / sb) / value,
                 "conv2_tflops": tf,
                 "conv2_frac_of_16bit_peak": tf / MFMA_PEAK_TFLOPS["bf16"],
+                "tile_variants": pb.variants,
+                "layer_tflops": layer_tflops(pb.net),
             }
             pb.net.close()
             del pb
@@ -404,6 +593,15 @@ def main() -> int:
         log("tsdf fusion")
         result["tsdf"] = tsdf_bench(torch)
         result["tsdf"]["config"] = "50 synthetic 640x480 depth frames into a 256^3 grid, 0.01 m voxels, truncation 0.05 m"
+        torch.cuda.empty_cache()
+        log("registration kernels")
+        result["registration_kernels"] = registration_kernels(torch, js, wt)
+        if not args.no_serving:
+            log("serving")
+            result["serving"] = serving_bench(torch, js, wt)
+            result["registration_latency_ms"] = {
+                "p50": result["serving"]["unloaded"]["p50_ms"],
+                "p99": result["serving"]["unloaded"]["p99_ms"]}
 
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, bounded sample)")

@@ -152,6 +152,29 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
                         const float origin[3], float voxel_size, float trunc_margin, int decode,
                         float *tdf, void *stream);
 
+/* Many clouds in one launch per step: the TDF of core/demo.cu:127-158 for a
+ * batch of fragments of different sizes, as a plan (made once, on the host)
+ * and runs (three launches for the whole batch: clear the bit grids, scatter
+ * every cloud's points, one tile kernel over every grid's tiles).  A run
+ * allocates nothing, sets no memory with memset and never waits on the host,
+ * so it can be captured into a hipGraph and replayed; it reads the point
+ * buffers named in the plan at run time.  Results are bit-identical to
+ * tdm_tdf_from_points per cloud. */
+typedef struct tdm_cloud_ref {
+    const float *pts;  /* device, num_pts*3 floats                         */
+    float *tdf;        /* device, dim_x*dim_y*dim_z floats (output)        */
+    int num_pts;
+    int dim_x, dim_y, dim_z;
+    float origin_x, origin_y, origin_z;
+    int reserved;
+} tdm_cloud_ref;
+typedef struct tdm_tdf_plan tdm_tdf_plan;
+/* clouds: host array of num entries (copied). */
+int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, float voxel_size,
+                        float trunc_margin, int decode, tdm_tdf_plan **out);
+int tdm_tdf_plan_run(tdm_tdf_plan *plan, void *stream);
+int tdm_tdf_plan_destroy(tdm_tdf_plan *plan);
+
 /* Keypoint grid coordinates round((p - origin)/voxel_size) (core/demo.cu:
  * 174-176).  kp: host num*3 floats; grid_out: host num*3 ints. */
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
@@ -208,6 +231,18 @@ int tdm_net_forward(tdm_net *net, const float *data, int num, float *feat, void 
  * host gather and the per-batch copies).  tdf, kp_grid, feat: device. */
 int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, int dim_z,
                          const int *kp_grid, int num, float *feat, void *stream);
+
+/* Tile autotuner (no counterpart in the reference, whose convolutions are
+ * cuDNN's, core/marvin.hpp:4635-4709): times every tile variant of each
+ * convolution's implicit-GEMM kernel on passes of `num` patches (capped at
+ * max_batch) and keeps the fastest per layer.  All variants give
+ * bit-identical results, so tuning changes speed only.  Overwrites the net's
+ * activation buffers; synchronizes `stream`. */
+int tdm_net_autotune(tdm_net *net, int num, void *stream);
+/* The tile variant of layer `index` (-1: default; -2: bad index), and a way
+ * to set it (-1 restores the default). */
+int tdm_net_layer_variant(const tdm_net *net, int index);
+int tdm_net_set_layer_variant(tdm_net *net, int index, int variant);
 
 /* One TDF grid of a multi-grid pass: device pointer and extents. */
 typedef struct tdm_grid_ref {

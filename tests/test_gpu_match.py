@@ -199,3 +199,17 @@ def test_ransac_small_sets(torch_gpu):
     rt, inl, st, _ = _ransac_gpu(torch_gpu, x, y, 100, 1)
     assert st[3] == 0 and st[0] == 3 and list(inl) == [0, 1, 2] and st[2] == 0
     assert np.abs(rt - ro.estimate_rt(x, y)).max() < 1e-10
+
+
+def test_nn_common_offset_and_dense_clusters(torch_gpu):
+    """Descriptors sharing a large common component (centring makes the
+    fp32 pass decisive) and a dense cluster of near-equidistant candidates
+    (the fp64 tile search decides)."""
+    rng = np.random.default_rng(4)
+    base = rng.normal(size=512).astype(np.float32) * 40
+    A = base + rng.normal(scale=0.5, size=(600, 512)).astype(np.float32)
+    B = base + rng.normal(scale=0.5, size=(700, 512)).astype(np.float32)
+    B[:200] = A[:200] + rng.normal(scale=0.05, size=(200, 512)).astype(np.float32)
+    B[650:] = B[600] + rng.normal(scale=1e-4, size=(50, 512)).astype(np.float32)
+    _, _, refined = _check_nn(torch_gpu, A, B)
+    print("refined queries", refined, flush=True)

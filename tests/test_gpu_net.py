@@ -174,3 +174,29 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
         net.forward_grids(table, torch.from_numpy(np.ascontiguousarray(kp_all[perm])).to("cuda:0"),
                           torch.from_numpy(order[perm].astype(np.int32)).to("cuda:0"), got)
         assert torch.equal(got, want), prec
+
+
+def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches):
+    """Every tile variant of every convolution gives the same descriptors bit
+    for bit, so the autotuner changes speed only."""
+    torch = torch_gpu
+    js, wt, _ = net_files
+    p = torch.from_numpy(patches[0]).to("cuda:0")
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
+        net = tdm.Net(js, prec, max_batch=4)
+        net.load_weights(wt)
+        ref = torch.empty(len(p), 512, dtype=torch.float32, device="cuda:0")
+        net.forward(p, ref)
+        for v in range(6):
+            for i in range(1, len(net.layer_names)):
+                net.set_layer_variant(i, v)  # variants that do not fit fall back
+            got = torch.empty_like(ref)
+            net.forward(p, got)
+            assert torch.equal(got, ref), (prec, v)
+        chosen = net.autotune(4)
+        print("autotuned variants", prec, chosen, flush=True)
+        assert all(-1 <= c < 6 for c in chosen)
+        got = torch.empty_like(ref)
+        net.forward(p, got)
+        assert torch.equal(got, ref)
+        net.close()
