@@ -410,25 +410,28 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
 // fetch each A row for more columns.  Every variant feeds each output the
 // same k in the same order (lane group g reads bytes 16g + 64h of a stage),
 // so all variants give bit-identical results and can be picked by timing.
-constexpr int kStageBytes = 128;
-constexpr int kLdsRow = 144;
-
-template <int WMW, int WTM, int WTN>
+// A stage is 128 or 64 bytes of k per row; LDS rows are padded by 16 bytes.
+// Both stage sizes walk k in the same order (64-byte halves in sequence).
+template <int WMW, int WTM, int WTN, int STAGE = 128>
 struct Tile {
     static constexpr int WNW = 4 / WMW;
     static constexpr int BM = WMW * WTM, BN = WNW * WTN;
-    static constexpr int LDS = 2 * (BM + BN) * kLdsRow;
+    static constexpr int ROW = STAGE + 16;
+    static constexpr int LDS = 2 * (BM + BN) * ROW;
     static constexpr int OCC = LDS <= 80 * 1024 ? 2 : 1;  // workgroups per CU
 };
 
-template <typename T, typename OT, int WMW, int WTM, int WTN, bool POOL, bool SPLIT>
-__global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kernel(ConvArgs p) {
-    using TL = Tile<WMW, WTM, WTN>;
+template <typename T, typename OT, int WMW, int WTM, int WTN, int STAGE, bool POOL, bool SPLIT>
+__global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_lds_kernel(ConvArgs p) {
+    using TL = Tile<WMW, WTM, WTN, STAGE>;
     constexpr int BM = TL::BM, BN = TL::BN, WNW = TL::WNW;
+    constexpr int kLdsRow = TL::ROW;
+    constexpr int PCS = STAGE / 16;                   // 16-byte pieces per staged row
+    constexpr int HALVES = STAGE / 64;
     constexpr int IB = WTM / 16, JB = WTN / 16;
-    constexpr int SK = kStageBytes / (int)sizeof(T);  // k per stage
-    constexpr int A_PT = BM * 8 / 256;                // 16-byte pieces per thread
-    constexpr int B_PT = BN * 8 / 256;
+    constexpr int SK = STAGE / (int)sizeof(T);        // k per stage
+    constexpr int A_PT = BM * PCS / 256;              // 16-byte pieces per thread
+    constexpr int B_PT = BN * PCS / 256;
     __shared__ __attribute__((aligned(16))) unsigned char lds[2][(BM + BN) * kLdsRow];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kern
     int a_dst[A_PT];
 #pragma unroll
     for (int j = 0; j < A_PT; ++j) {
-        const int idx = tid + 256 * j, row = idx >> 3, pc = idx & 7;
+        const int idx = tid + 256 * j, row = idx / PCS, pc = idx % PCS;
         const long m = min(mb + row, p.M - 1);
         a_src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + pc * 16;
         a_dst[j] = row * kLdsRow + pc * 16;
@@ -455,7 +458,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kern
     int b_dst[B_PT];
 #pragma unroll
     for (int j = 0; j < B_PT; ++j) {
-        const int idx = tid + 256 * j, col = idx >> 3, pc = idx & 7;
+        const int idx = tid + 256 * j, col = idx / PCS, pc = idx % PCS;
         b_src[j] = w + ((long)(nb + col) * K) * (long)sizeof(T) + pc * 16;
         b_dst[j] = (BM + col) * kLdsRow + pc * 16;
     }
@@ -464,8 +467,12 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kern
     const int ns = p.taps * cps;
     int s0 = 0, s1 = ns;
     if (SPLIT) {
-        s0 = (int)((long)ns * tz / p.splits);
-        s1 = (int)((long)ns * (tz + 1) / p.splits);
+        // split points in 128-byte units of k, whatever the stage size, so
+        // every variant sums the same k ranges into each partial
+        constexpr int CG = 128 / STAGE;
+        const int nc = ns / CG;
+        s0 = (int)((long)nc * tz / p.splits) * CG;
+        s1 = (int)((long)nc * (tz + 1) / p.splits) * CG;
     }
     uint4 ra[A_PT], rb[B_PT];
     auto fetch = [&](int s) {
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN>::OCC)) void conv_lds_kern
             const unsigned char *A = &lds[buf][(wm * WTM + r16) * kLdsRow + g * 16];
             const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < HALVES; ++h) {
                 uint4 a[IB], b[JB];
 #pragma unroll
                 for (int i = 0; i < IB; ++i)
@@ -790,24 +797,27 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     if (lds) {
         int v = variant;
         if (v < 0 || !tdm::conv_variant_valid(v, d)) v = tdm::conv_default_variant(d);
-#define TDM_CONV_LDS(WMW, WTM, WTN)                                                           \
+#define TDM_CONV_LDS(WMW, WTM, WTN, ST)                                                       \
     do {                                                                                      \
-        using TL = Tile<WMW, WTM, WTN>;                                                       \
+        using TL = Tile<WMW, WTM, WTN, ST>;                                                   \
         dim3 grid((unsigned)((p.M + TL::BM - 1) / TL::BM), d.N / TL::BN, splits);             \
         if (splits > 1)                                                                       \
-            conv_lds_kernel<T, OT, WMW, WTM, WTN, false, true><<<grid, 256, 0, s>>>(p);       \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, ST, false, true><<<grid, 256, 0, s>>>(p);   \
         else if (d.pool)                                                                      \
-            conv_lds_kernel<T, OT, WMW, WTM, WTN, true, false><<<grid, 256, 0, s>>>(p);       \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, ST, true, false><<<grid, 256, 0, s>>>(p);   \
         else                                                                                  \
-            conv_lds_kernel<T, OT, WMW, WTM, WTN, false, false><<<grid, 256, 0, s>>>(p);      \
+            conv_lds_kernel<T, OT, WMW, WTM, WTN, ST, false, false><<<grid, 256, 0, s>>>(p);  \
     } while (0)
         switch (v) {
-        case 0: TDM_CONV_LDS(2, 64, 32); break;
-        case 1: TDM_CONV_LDS(2, 64, 64); break;
-        case 2: TDM_CONV_LDS(4, 64, 64); break;
-        case 3: TDM_CONV_LDS(2, 128, 32); break;
-        case 4: TDM_CONV_LDS(2, 128, 64); break;
-        default: TDM_CONV_LDS(1, 64, 64); break;
+        case 0: TDM_CONV_LDS(2, 64, 32, 128); break;
+        case 1: TDM_CONV_LDS(2, 64, 64, 128); break;
+        case 2: TDM_CONV_LDS(4, 64, 64, 128); break;
+        case 3: TDM_CONV_LDS(2, 128, 32, 128); break;
+        case 4: TDM_CONV_LDS(2, 128, 64, 128); break;
+        case 5: TDM_CONV_LDS(1, 64, 64, 128); break;
+        case 6: TDM_CONV_LDS(4, 64, 64, 64); break;
+        case 7: TDM_CONV_LDS(2, 64, 64, 64); break;
+        default: TDM_CONV_LDS(2, 128, 32, 64); break;
         }
 #undef TDM_CONV_LDS
     } else {
@@ -864,8 +874,9 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
 
 // Tile variants of the LDS kernel: {waves along m, wave tile m, wave tile n}.
 namespace {
-constexpr int kVariants[kConvVariants][3] = {{2, 64, 32}, {2, 64, 64}, {4, 64, 64},
-                                             {2, 128, 32}, {2, 128, 64}, {1, 64, 64}};
+constexpr int kVariants[kConvVariants][3] = {{2, 64, 32},  {2, 64, 64},  {4, 64, 64},
+                                             {2, 128, 32}, {2, 128, 64}, {1, 64, 64},
+                                             {4, 64, 64},  {2, 64, 64},  {2, 128, 32}};
 }
 
 bool conv_variant_valid(int v, const ConvDesc &d) {

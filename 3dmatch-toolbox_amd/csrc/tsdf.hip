@@ -161,10 +161,23 @@ __global__ __launch_bounds__(kThreads) void integrate_frames_kernel(
     const float *__restrict__ K, const float *__restrict__ cam2base,
     const float *__restrict__ depths, int F, int H, int W, int dx, int dy, int dz, float ox,
     float oy, float oz, float vs, float trunc, float *__restrict__ tsdf,
-    float *__restrict__ weight) {
+    float *__restrict__ weight, const float *__restrict__ origins) {
     const long plane = (long)dx * dy;
     const long col = (long)blockIdx.x * kThreads + threadIdx.x;
     if (col >= plane) return;
+    // scene blockIdx.z of a batch: its own poses, frames, grid and origin
+    if (gridDim.z > 1 || origins) {
+        const long sc = blockIdx.z;
+        cam2base += sc * F * 16;
+        depths += sc * F * (long)H * W;
+        tsdf += sc * plane * dz;
+        weight += sc * plane * dz;
+        if (origins) {
+            ox = origins[3 * sc + 0];
+            oy = origins[3 * sc + 1];
+            oz = origins[3 * sc + 2];
+        }
+    }
     const int x = (int)(col % dx);
     const int y = (int)(col / dx);
     const float bx = ox + x * vs;
@@ -386,7 +399,30 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
     integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
         cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z,
-        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight);
+        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight, nullptr);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_integrate_scenes(tdm_ctx *ctx, const float *cam_K, const float *cam2base,
+                         const float *depth_ims, int num_scenes, int num_frames, int im_height,
+                         int im_width, int dim_x, int dim_y, int dim_z, const float *origins,
+                         float voxel_size, float trunc_margin, float *tsdf, float *weight,
+                         void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(cam_K && cam2base && depth_ims && origins && tsdf && weight, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_CHECK(num_frames >= 0 && num_scenes >= 0 && num_scenes < 65536, TDM_ERR_INVALID_ARG,
+              "bad scene/frame count");
+    TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    if (num_frames == 0 || num_scenes == 0) return TDM_OK;
+    long plane = (long)dim_x * dim_y;
+    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRunF - 1) / kZRunF, num_scenes);
+    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
+    integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+        cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z, 0.0f,
+        0.0f, 0.0f, voxel_size, trunc_margin, tsdf, weight, origins);
     TDM_LAUNCHED();
     return TDM_OK;
 }

@@ -125,6 +125,17 @@ def integrate_frames(cam_K, cam2base, depths, dims, origin, voxel_size, trunc, t
                                       _stream(stream)))
 
 
+def integrate_scenes(cam_K, cam2base, depths, dims, origins, voxel_size, trunc, tsdf, weight,
+                     stream=None):
+    """Many scenes in one launch.  cam2base [S, F, 16]; depths [S, F, H, W];
+    origins [S, 3]; tsdf/weight [S, N] (all device)."""
+    S, F, H, W = depths.shape
+    _check(lib().tdm_integrate_scenes(None, _p(cam_K), _p(cam2base), _p(depths), c_int(S), c_int(F),
+                                      c_int(H), c_int(W), c_int(dims[0]), c_int(dims[1]),
+                                      c_int(dims[2]), _p(origins), _f(voxel_size), _f(trunc),
+                                      _p(tsdf), _p(weight), _stream(stream)))
+
+
 def surface_points(tsdf, weight, dims, origin, voxel_size, pts=None, capacity=0,
                    tsdf_thresh=0.2, weight_thresh=1.0, stream=None) -> int:
     """GPU surface extraction (depth-fusion/utils.hpp:14-56) into pts

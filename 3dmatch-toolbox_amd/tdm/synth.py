@@ -190,3 +190,61 @@ def fragment_pair(seed: int, n1: int | None = None, n2: int | None = None,
     t = d / np.linalg.norm(d) * rng.uniform(0.0, max_shift)
     pts2 = (out[1] - t) @ R  # R^T (x - t) for row vectors
     return out[0].astype(np.float32), pts2.astype(np.float32), R, t
+
+
+def cropped_fragment(seed: int, n_points: int, extent: float, sigma: float = 0.002,
+                     scene_seed: int = 1) -> np.ndarray:
+    """[n_points, 3] float32 cloud: the points of a random view inside a cube
+    of side `extent` around a random surface point, resampled to n_points
+    with N(0, sigma^2) noise (fragments of the batched workload)."""
+    rng = np.random.default_rng(seed)
+    P = _view_points(int(rng.integers(0, 50)), scene_seed)
+    for _ in range(20):
+        c = P[rng.integers(0, len(P))]
+        keep = np.all(np.abs(P - c) <= extent / 2, axis=1)
+        if keep.sum() >= 2000:
+            break
+    part = P[keep]
+    idx = rng.choice(len(part), size=n_points, replace=len(part) < n_points)
+    return (part[idx] + rng.standard_normal((n_points, 3)) * sigma).astype(np.float32)
+
+
+def depth_frames_torch(torch, n: int = 50, scene_seed: int = 1, noise_seed: int = 2,
+                       dropout: float = 0.05, device="cuda:0"):
+    """depth_frames on the GPU with torch (same scene, cameras, noise model
+    and millimetre quantisation; its own random stream): (depth in metres
+    [n, H, W] float32 as tdm_depth_from_mm would give, cam2world [n, 4, 4]
+    float64 numpy).  For large synthetic batches."""
+    balls = torch.tensor(spheres(scene_seed), dtype=torch.float64, device=device)
+    poses = camera_poses(n)
+    rays = torch.tensor(_pixel_rays(), dtype=torch.float64, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(noise_seed)
+    lo = torch.tensor(ROOM_LO, dtype=torch.float64, device=device)
+    hi = torch.tensor(ROOM_HI, dtype=torch.float64, device=device)
+    out = torch.empty(n, H, W, dtype=torch.float32, device=device)
+    for i in range(n):
+        R = torch.tensor(poses[i, :3, :3], dtype=torch.float64, device=device)
+        c = torch.tensor(poses[i, :3, 3], dtype=torch.float64, device=device)
+        d = rays @ R.T
+        big = torch.full_like(d, float("inf"))
+        t_hi = torch.where(d > 0, (hi - c) / d, torch.where(d < 0, (lo - c) / d, big))
+        t = t_hi.min(dim=1).values
+        dd = (d * d).sum(1)
+        for s in balls:
+            oc = c - s[:3]
+            b = d @ oc
+            disc = b * b - dd * ((oc * oc).sum() - s[3] ** 2)
+            ok = disc >= 0
+            sq = torch.sqrt(torch.clamp(disc, min=0.0))
+            t0 = (-b - sq) / dd
+            t1 = (-b + sq) / dd
+            hit = torch.where(t0 > 1e-6, t0, torch.where(t1 > 1e-6, t1, torch.full_like(t0, float("inf"))))
+            t = torch.where(ok, torch.minimum(t, hit), t)
+        z = t.reshape(H, W)
+        z = z + torch.randn(z.shape, generator=gen, device=device, dtype=torch.float64) * (0.0015 * z * z)
+        z = torch.where(torch.rand(z.shape, generator=gen, device=device) < dropout, torch.zeros_like(z), z)
+        mm = torch.clamp(torch.round(z * 1000.0), 0, 65535)
+        m = (mm.float() / 1000.0)
+        out[i] = torch.where(m > 6.0, torch.zeros_like(m), m)
+    return out, poses

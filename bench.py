@@ -169,6 +169,51 @@ def tsdf_bench(torch, reps=3):
     return out
 
 
+def tsdf_scenes_bench(torch, n_scenes=16, reps=3):
+    """Batched TSDF: n_scenes synthetic scenes (sphere seeds 50..), 50 frames
+    each, fused into n_scenes 256^3 grids in one launch."""
+    import numpy as np
+
+    import tdm
+    from tdm import synth
+
+    dev = torch.device("cuda:0")
+    F = 50
+    depths = []
+    for sc in range(n_scenes):
+        d, c2w = synth.depth_frames_torch(torch, F, scene_seed=50 + sc, noise_seed=100 + sc)
+        depths.append(d)
+    D = torch.stack(depths)
+    poses = synth.relative_poses(c2w)  # the camera circle is the same in every scene
+    P = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(
+        poses.reshape(1, F, 16), (n_scenes, F, 16)))).to(dev)
+    origins = torch.from_numpy(np.tile(synth.TSDF_ORIGIN, (n_scenes, 1))).to(dev)
+    K = torch.from_numpy(synth.intrinsics()).to(dev)
+    dims = synth.TSDF_DIMS
+    n = dims[0] * dims[1] * dims[2]
+    tsdf = torch.empty(n_scenes * n, device=dev)
+    weight = torch.empty(n_scenes * n, device=dev)
+    times = []
+    for _ in range(reps + 1):
+        tdm.tsdf_init(tsdf, weight)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        tdm.integrate_scenes(K, P, D, dims, origins, synth.TSDF_VOXEL, synth.TSDF_TRUNC, tsdf,
+                             weight)
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times[1:]))
+    touched = float((weight > 0).sum().item())
+    H, W = D.shape[-2:]
+    algo = 16.0 * touched + n_scenes * F * 4.0 * H * W
+    return {"scenes": n_scenes, "frames": F, "ms": ms,
+            "voxel_updates_per_s": n_scenes * n * F / (ms / 1e3),
+            "applied_updates": float(weight.sum().item()),
+            "hbm_gbs": algo / (ms / 1e3) / 1e9,
+            "config": "16 scenes x 50 frames, 640x480, into 16 grids of 256^3 in one launch"}
+
+
 def mutual_nn(torch, a, b):
     """Exact nearest neighbours of a's rows in b and the mutual mask, with the
     library's matcher (the knnsearch pair of core/demo.m:49-55)."""
@@ -421,6 +466,88 @@ def serving_bench(torch, js, wt, rates=(2.0, 4.0, 8.0), per_rate=16, unloaded=8)
     return out
 
 
+def batched_bench(torch, js, wt, n_frag=64, seed=40, replays=2):
+    """Batched form: n_frag fragments (points U[20k, 500k], extents U[1, 4] m,
+    keypoints U[1000, 10000]) -> one TDF plan (three launches for all grids)
+    -> one multi-grid fp16 descriptor pass over every keypoint, the whole
+    batch captured in a hipGraph and replayed.  Reports descriptors/s of a
+    replay and of the same work issued eagerly."""
+    import numpy as np
+
+    import tdm
+    from tdm import synth
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    clouds, kps, gids = [], [], []
+    total_pts = 0
+    for f in range(n_frag):
+        n = int(rng.integers(20_000, 500_001))
+        cloud = synth.cropped_fragment(seed * 1000 + f, n, float(rng.uniform(1.0, 4.0)))
+        pts = torch.from_numpy(cloud).to(dev)
+        dims, origin = tdm.grid_params(pts, 0.01, 15)
+        tdf = torch.empty(dims[2], dims[1], dims[0], device=dev)
+        clouds.append((pts, tdf, dims, origin))
+        k = int(rng.integers(1000, 10_001))
+        idx = rng.integers(0, n, k)
+        kps.append(tdm.keypoints_to_grid(cloud[idx], origin, np.float32(0.01)))
+        gids.append(np.full(k, f, np.int32))
+        total_pts += n
+    kp = torch.from_numpy(np.concatenate(kps)).to(dev)
+    gid = torch.from_numpy(np.concatenate(gids)).to(dev)
+    n_kp = len(kp)
+    plan = tdm.TdfPlan(clouds)
+    table = tdm.grid_table([(tdf, dims) for _, tdf, dims, _ in clouds])
+    net = tdm.Net(js, tdm.PREC_FP16, max_batch=1024)
+    net.load_weights(wt)
+    variants = net.autotune(1024)
+    feat = torch.empty(n_kp, 512, device=dev)
+
+    def step():
+        plan.run()
+        net.forward_grids(table, kp, gid, feat)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    eager = time.perf_counter() - t0
+    want = feat.clone()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            step()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    same = bool(torch.equal(feat, want))
+    t0 = time.perf_counter()
+    for _ in range(replays):
+        g.replay()
+    torch.cuda.synchronize()
+    graph = (time.perf_counter() - t0) / replays
+    # TDF part alone (plan run) for its own rate
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    plan.run()
+    e1.record()
+    torch.cuda.synchronize()
+    tdf_ms = e0.elapsed_time(e1)
+    voxels = sum(d[0] * d[1] * d[2] for _, _, d, _ in clouds)
+    plan.close()
+    net.close()
+    return {"fragments": n_frag, "points": total_pts, "keypoints": n_kp, "voxels": voxels,
+            "eager_s": eager, "graph_s": graph,
+            "descriptors_per_s_graph": n_kp / graph, "descriptors_per_s_eager": n_kp / eager,
+            "graph_equals_eager": same, "tdf_plan_ms": tdf_ms,
+            "tdf_gvoxels_per_s": voxels / (tdf_ms / 1e3) / 1e9, "tile_variants": variants,
+            "config": ("64 fragments, points U[20k,500k], extents U[1,4] m, keypoints "
+                       "U[1000,10000], fp16 net, TDF plan + multi-grid pass in one hipGraph")}
+
+
 def cpu_baseline(cloud, keypts, weights, n_kp, conv_patches=32):
     """Oracle (C loop nests, OpenMP) on a bounded sample of the descriptor
     workload: TDF of 1/8 of the grid's voxels and `conv_patches` patches
@@ -593,9 +720,15 @@ def main() -> int:
         log("tsdf fusion")
         result["tsdf"] = tsdf_bench(torch)
         result["tsdf"]["config"] = "50 synthetic 640x480 depth frames into a 256^3 grid, 0.01 m voxels, truncation 0.05 m"
+        log("tsdf scenes batch")
+        result["tsdf"]["scenes_batch"] = tsdf_scenes_bench(torch)
+        torch.cuda.empty_cache()
         torch.cuda.empty_cache()
         log("registration kernels")
         result["registration_kernels"] = registration_kernels(torch, js, wt)
+        log("batched form (hipGraph)")
+        result["batched"] = batched_bench(torch, js, wt)
+        torch.cuda.empty_cache()
         if not args.no_serving:
             log("serving")
             result["serving"] = serving_bench(torch, js, wt)

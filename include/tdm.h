@@ -105,6 +105,18 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
                          float origin_z, float voxel_size, float trunc_margin, float *tsdf,
                          float *weight, void *stream);
 
+/* num_scenes independent fusions in one launch (the frame loop of
+ * depth-fusion/demo.cu:156-183 for a batch of scenes sharing intrinsics,
+ * image size and grid extent): scene s uses poses cam2base[s*F*16 ..],
+ * frames depth_ims[s*F*H*W ..], origin origins[3s .. 3s+2] and the grids
+ * tsdf/weight[s*N ..] (N = dim_x*dim_y*dim_z).  All device.  Each scene's
+ * grids are bit-identical to tdm_integrate_frames on that scene. */
+int tdm_integrate_scenes(tdm_ctx *ctx, const float *cam_K, const float *cam2base,
+                         const float *depth_ims, int num_scenes, int num_frames, int im_height,
+                         int im_width, int dim_x, int dim_y, int dim_z, const float *origins,
+                         float voxel_size, float trunc_margin, float *tsdf, float *weight,
+                         void *stream);
+
 /* Surface points of a fused grid on the GPU: what
  * SaveVoxelGrid2SurfacePointCloud (depth-fusion/utils.hpp:14-56) writes to
  * tsdf.ply -- every voxel with |tsdf| < tsdf_thresh and weight >

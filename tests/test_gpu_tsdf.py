@@ -184,3 +184,31 @@ def test_surface_points_bit_exact(torch_gpu, synth_frames):
     # nothing passes: empty set
     n, _ = surface_gpu(torch_gpu, np.ones_like(t), w, dims, origin, vs)
     assert n == 0
+
+
+def test_integrate_scenes_equals_per_scene(torch_gpu, synth_frames):
+    """Three scenes (different poses/frames/origins) in one launch equal three
+    tdm_integrate_frames calls bit for bit."""
+    torch = torch_gpu
+    depths, poses, K = synth_frames
+    dims = (72, 60, 56)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    origins = np.array([[-0.4, -0.3, 1.0], [-0.3, -0.35, 1.2], [-0.5, -0.2, 0.9]], np.float32)
+    order = [[0, 1, 2, 3], [3, 2, 1, 0], [1, 3, 0, 2]]
+    D = np.stack([depths[o] for o in order])
+    P = np.stack([poses[o] for o in order]).astype(np.float32)
+    dev = "cuda:0"
+    n = dims[0] * dims[1] * dims[2]
+    t = torch.empty(3 * n, device=dev)
+    w = torch.empty(3 * n, device=dev)
+    tdm.tsdf_init(t, w)
+    dK = torch.from_numpy(np.ascontiguousarray(K, np.float32)).to(dev)
+    tdm.integrate_scenes(dK, torch.from_numpy(P.reshape(3, 4, 16)).to(dev),
+                         torch.from_numpy(np.ascontiguousarray(D)).to(dev), dims,
+                         torch.from_numpy(origins).to(dev), vs, trunc, t, w)
+    torch.cuda.synchronize()
+    for s in range(3):
+        ref = _gpu_fuse(torch, K, P[s], D[s], dims, origins[s], vs, trunc, True)
+        got = (t[s * n:(s + 1) * n].cpu().numpy().reshape(ref[0].shape),
+               w[s * n:(s + 1) * n].cpu().numpy().reshape(ref[0].shape))
+        _assert_same(got, ref)
