@@ -601,6 +601,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "bf16", "fp16"], default="fp32",
+                    help="precision of the headline pipeline (default fp32, the plain path)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving sweep (e.g. under a profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -639,8 +641,9 @@ def main() -> int:
     netdef.write_net_json(js)
     tdm.marvin_write(wt, weights)
 
-    pipe = DescriptorPipeline(torch, cloud, keypts, js, wt, tdm.PREC_FP32, args.batch)
-    log(f"fp32 descriptor pipeline: grid {pipe.dims}, batch {args.batch}")
+    prec_id = {"fp32": tdm.PREC_FP32, "bf16": tdm.PREC_BF16, "fp16": tdm.PREC_FP16}[args.precision]
+    pipe = DescriptorPipeline(torch, cloud, keypts, js, wt, prec_id, args.batch)
+    log(f"{args.precision} descriptor pipeline: grid {pipe.dims}, batch {args.batch}")
     pipe.net.set_profiling(True)
     sec = time_steps(torch, pipe.step, args.steps, args.warmup)
     value = n_kp / sec
@@ -655,7 +658,7 @@ def main() -> int:
     conv2_flops = 2 * 26 ** 3 * 64 * 64 * 27
     achieved = conv2_flops * patches2 / (ms2 / 1e3) / 1e12
     pipe.net.set_profiling(False)
-    log(f"fp32: {value:.1f} descriptors/s, conv2 {achieved:.1f} TFLOP/s")
+    log(f"{args.precision}: {value:.1f} descriptors/s, conv2 {achieved:.1f} TFLOP/s")
 
     result = {
         "metric": METRIC,
@@ -668,27 +671,28 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": args.precision,
         "data": "synthetic",
         "config": {
-            "workload": ("point cloud -> TDF -> 3DMatch descriptors, fp32: 200k-point synthetic "
+            "workload": (f"point cloud -> TDF -> 3DMatch descriptors, {args.precision}: 200k-point synthetic "
                          "fragment, 0.01 m voxels, truncation 5 voxels, 5000 keypoints, 30^3 "
                          "patches, He-normal weights"),
             "points": len(cloud), "keypoints": n_kp, "grid": pipe.dims, "batch": args.batch,
             "tile_variants": pipe.variants,
         },
         "roofline": {
-            "kernel": "conv2: fp32 MFMA implicit GEMM 64->64 ch, 28^3->26^3, fused bias+ReLU+2^3 max pool",
+            "kernel": (f"conv2: {args.precision} MFMA implicit GEMM 64->64 ch, 28^3->26^3, "
+                       "fused bias+ReLU+2^3 max pool"),
             "bound": "mfma",
             "achieved": achieved,
-            "peak": MFMA_PEAK_TFLOPS["fp32"],
+            "peak": MFMA_PEAK_TFLOPS[args.precision],
             "unit": "TFLOP/s",
-            "frac": achieved / MFMA_PEAK_TFLOPS["fp32"],
+            "frac": achieved / MFMA_PEAK_TFLOPS[args.precision],
             "traffic": None,
             "flop_per_launch": conv2_flops * (patches2 / max(passes2, 1)),
             "avg_launch_ms": ms2 / max(passes2, 1),
         },
-        "layers_fp32": layers,
+        "layers": layers,
     }
     pipe.net.close()
     del pipe
