@@ -37,6 +37,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t bf16_t;
 typedef _Float16 f16_t;
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned char fp8_t;  // e4m3, the encoding of gfx950's fp8 conversions and MFMA
 
 template <typename T>
 struct Kc;
@@ -98,6 +99,28 @@ struct Mfma<bf16_t> {
     }
 };
 
+// fp8: a 16-byte fragment is two 8-byte k-groups, one v_mfma_f32_16x16x32_fp8
+// step each (lane group g supplies 8 consecutive k of a 32-k step).
+template <>
+struct Mfma<fp8_t> {
+    template <int IB, int JB>
+    __device__ static inline void chunk(floatx4 (&acc)[IB][JB], const uint4 (&a)[IB],
+                                        const uint4 (&b)[JB]) {
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+#pragma unroll
+                for (int j = 0; j < JB; ++j) {
+                    const long av = hh == 0 ? (long)(((unsigned long)a[i].y << 32) | a[i].x)
+                                            : (long)(((unsigned long)a[i].w << 32) | a[i].z);
+                    const long bv = hh == 0 ? (long)(((unsigned long)b[j].y << 32) | b[j].x)
+                                            : (long)(((unsigned long)b[j].w << 32) | b[j].z);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, bv, acc[i][j], 0, 0, 0);
+                }
+    }
+};
+
 template <>
 struct Mfma<f16_t> {
     template <int IB, int JB>
@@ -138,6 +161,17 @@ template <>
 __device__ inline void store<f16_t>(f16_t *p, float v) {
     *p = (f16_t)v;  // round to nearest even, overflow to inf
 }
+// Saturating round-to-nearest-even conversion to e4m3 (the caller has
+// applied the activation scale).
+__device__ inline fp8_t to_fp8(float v) {
+    v = fminf(fmaxf(v, -tdm::kFp8Max), tdm::kFp8Max);
+    return (fp8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(v, v, 0, false) & 0xff);
+}
+__device__ inline float from_fp8(fp8_t b) { return __builtin_amdgcn_cvt_f32_fp8((int)b, 0); }
+template <>
+__device__ inline void store<fp8_t>(fp8_t *p, float v) {
+    *p = to_fp8(v);
+}
 
 // 16-bit storage bits of a float for the MFMA operand types.
 template <typename T>
@@ -156,8 +190,9 @@ __device__ inline unsigned short bits16<f16_t>(float v) {
 
 template <typename T>
 constexpr tdm::DType dtype_of() {
-    return sizeof(T) == 4 ? tdm::DType::F32
-                          : (__is_same(T, f16_t) ? tdm::DType::F16 : tdm::DType::BF16);
+    return sizeof(T) == 4   ? tdm::DType::F32
+           : sizeof(T) == 1 ? tdm::DType::F8
+                            : (__is_same(T, f16_t) ? tdm::DType::F16 : tdm::DType::BF16);
 }
 
 struct ConvArgs {
@@ -169,6 +204,8 @@ struct ConvArgs {
     long M;  // rows
     int N, C, D, OD, ksize, taps, splits;
     int relu;
+    const float *colscale;  // fp8: per output channel dequantization scale (else NULL)
+    float oscale;           // fp8 outputs: multiplier before conversion (else 1)
 };
 
 // Workgroup -> tile order that keeps neighbouring tiles on one XCD.  The
@@ -371,17 +408,20 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
         return;
     }
     OT *__restrict__ out = static_cast<OT *>(p.out);
+    constexpr bool Q8 = sizeof(OT) == 1;
 #pragma unroll
     for (int j = 0; j < JB; ++j) {
         const int col = n0 + 16 * j + r16;
         const float bv = p.bias[col];
+        const float cs = p.colscale ? p.colscale[col] : 1.0f;
 #pragma unroll
         for (int i = 0; i < IB; ++i) {
             float v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                v[r] = acc[i][j][r] + bv;
+                v[r] = p.colscale ? acc[i][j][r] * cs + bv : acc[i][j][r] + bv;
                 if (p.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+                if (Q8) v[r] *= p.oscale;
             }
             if (POOL) {
                 float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
@@ -472,7 +512,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         constexpr int CG = 128 / STAGE;
         const int nc = ns / CG;
         s0 = (int)((long)nc * tz / p.splits) * CG;
-        s1 = (int)((long)nc * (tz + 1) / p.splits) * CG;
+        s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
     }
     uint4 ra[A_PT], rb[B_PT];
     auto fetch = [&](int s) {
@@ -536,13 +576,15 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
 template <typename OT>
 __global__ void split_reduce_kernel(const float *__restrict__ part, int splits, long MN, int N,
-                                    const float *__restrict__ bias, int relu, OT *__restrict__ out) {
+                                    const float *__restrict__ bias, int relu, OT *__restrict__ out,
+                                    const float *__restrict__ colscale, float oscale) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < MN;
          i += (long)gridDim.x * blockDim.x) {
         float s = part[i];
         for (int k = 1; k < splits; ++k) s += part[(long)k * MN + i];
-        s += bias[i % N];
+        s = colscale ? s * colscale[i % N] + bias[i % N] : s + bias[i % N];
         if (relu) s = s > 0.0f ? s : 0.0f;
+        if (sizeof(OT) == 1) s *= oscale;
         store<OT>(out + i, s);
     }
 }
@@ -562,6 +604,7 @@ struct Conv1Args {
     long M;
     int N;
     int relu;
+    float oscale;  // fp8 outputs: multiplier before conversion
 };
 
 // MODE 0: patches [batch][P^3]; 1: keypoints in one grid; 2: keypoints in
@@ -698,6 +741,7 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
             for (int r = 0; r < 4; ++r) {
                 float v = acc[i][j][r] + bv;
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
+                if (sizeof(OT) == 1) v *= a.oscale;
                 store<OT>(t + (16 * i + 4 * g + r) * PITCH + cl, v);
             }
     }
@@ -733,9 +777,14 @@ __device__ inline float load_as_float<f16_t>(const f16_t *p) {
     return (float)*p;
 }
 
+template <>
+__device__ inline float load_as_float<fp8_t>(const fp8_t *p) {
+    return from_fp8(*p);
+}
+
 template <typename T>
 __global__ void to_ncdhw_kernel(const T *__restrict__ in, long batch, int C, int E,
-                                float *__restrict__ out) {
+                                float *__restrict__ out, float scale) {
     const long vol = (long)E * E * E;
     const long total = batch * C * vol;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -744,7 +793,8 @@ __global__ void to_ncdhw_kernel(const T *__restrict__ in, long batch, int C, int
         const long bc = i / vol;
         const int c = (int)(bc % C);
         const long b = bc / C;
-        out[i] = load_as_float<T>(in + (b * vol + v) * C + c);
+        const float x = load_as_float<T>(in + (b * vol + v) * C + c);
+        out[i] = sizeof(T) == 1 ? x * scale : x;
     }
 }
 
@@ -777,8 +827,10 @@ __global__ void maxpool_kernel(const T *__restrict__ in, long batch, int C, int 
 template <typename T, typename OT>
 int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void *w,
                   const float *bias, void *out, int splits, float *partial, int variant,
-                  hipStream_t s) {
+                  hipStream_t s, const float *colscale, float oscale) {
     ConvArgs p;
+    p.colscale = colscale;
+    p.oscale = oscale;
     p.in = in;
     p.w = w;
     p.bias = bias;
@@ -796,7 +848,8 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     const bool lds = tdm::conv_uses_lds(dtype_of<T>(), d);
     if (lds) {
         int v = variant;
-        if (v < 0 || !tdm::conv_variant_valid(v, d)) v = tdm::conv_default_variant(d);
+        constexpr tdm::DType DT = dtype_of<T>();
+        if (v < 0 || !tdm::conv_variant_valid(DT, v, d)) v = tdm::conv_default_variant(DT, d);
 #define TDM_CONV_LDS(WMW, WTM, WTN, ST)                                                       \
     do {                                                                                      \
         using TL = Tile<WMW, WTM, WTN, ST>;                                                   \
@@ -820,6 +873,8 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         default: TDM_CONV_LDS(2, 128, 32, 64); break;
         }
 #undef TDM_CONV_LDS
+    } else if constexpr (sizeof(T) == 1) {
+        return tdm::fail(TDM_ERR_UNSUPPORTED, "fp8 convolution with %d input channels", d.C);
     } else {
         const int waves_n = d.N % 256 == 0 ? 4 : d.N % 128 == 0 ? 2 : 1;
         const int waves_m = 4 / waves_n;
@@ -844,7 +899,8 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         const long MN = p.M * d.N;
         unsigned blocks = (unsigned)std::min<long>((MN + 255) / 256, 16384);
         split_reduce_kernel<OT><<<blocks, 256, 0, s>>>(partial, splits, MN, d.N, bias, p.relu,
-                                                       static_cast<OT *>(out));
+                                                       static_cast<OT *>(out), p.colscale,
+                                                       p.oscale);
         TDM_LAUNCHED();
     }
     return TDM_OK;
@@ -862,30 +918,40 @@ bool conv_uses_lds(DType dt, const ConvDesc &d) {
         const char *e = std::getenv("TDM_CONV_IMPL");
         return e && std::strcmp(e, "direct") == 0;
     }();
-    const int sk = dt == DType::F32 ? 32 : 64;
-    return !direct && d.C % sk == 0 && d.N % 64 == 0;
+    // channels per 64-byte half stage
+    const int sk = dt == DType::F32 ? 16 : dt == DType::F8 ? 64 : 32;
+    if (dt == DType::F8) return d.C % sk == 0 && d.N % 64 == 0;
+    return !direct && d.C % (2 * sk) == 0 && d.N % 64 == 0;
 }
 
+// Reduction steps in 128-byte units of k (the unit split-K divides).
 int conv_reduction_steps(DType dt, const ConvDesc &d) {
     const int taps = d.ksize * d.ksize * d.ksize;
-    if (conv_uses_lds(dt, d)) return taps * d.C / (dt == DType::F32 ? 32 : 64);
+    if (conv_uses_lds(dt, d))
+        return std::max(1, taps * d.C / (dt == DType::F32 ? 32 : dt == DType::F8 ? 128 : 64));
     return taps * d.C / (dt == DType::F32 ? 16 : 32);
 }
 
-// Tile variants of the LDS kernel: {waves along m, wave tile m, wave tile n}.
+// Tile variants of the LDS kernel: {waves along m, wave tile m, wave tile n,
+// stage bytes}.
 namespace {
-constexpr int kVariants[kConvVariants][3] = {{2, 64, 32},  {2, 64, 64},  {4, 64, 64},
-                                             {2, 128, 32}, {2, 128, 64}, {1, 64, 64},
-                                             {4, 64, 64},  {2, 64, 64},  {2, 128, 32}};
+constexpr int kVariants[kConvVariants][4] = {
+    {2, 64, 32, 128},  {2, 64, 64, 128}, {4, 64, 64, 128}, {2, 128, 32, 128}, {2, 128, 64, 128},
+    {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64}};
 }
 
-bool conv_variant_valid(int v, const ConvDesc &d) {
+bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     if (v < 0 || v >= kConvVariants) return false;
     const int bn = (4 / kVariants[v][0]) * kVariants[v][2];
-    return d.N % bn == 0;
+    const int stage_k = kVariants[v][3] / (int)dtype_size(dt);
+    return d.N % bn == 0 && d.C % stage_k == 0;
 }
 
-int conv_default_variant(const ConvDesc &d) { return d.N % 128 == 0 ? 1 : 0; }
+int conv_default_variant(DType dt, const ConvDesc &d) {
+    const int v = d.N % 128 == 0 ? 1 : 0;
+    if (conv_variant_valid(dt, v, d)) return v;
+    return d.N % 128 == 0 ? 7 : 6;  // 64-byte stages (fp8 with 64 channels)
+}
 
 void conv_variant_tile(int v, int *bm, int *bn) {
     *bm = kVariants[v][0] * kVariants[v][1];
@@ -909,7 +975,7 @@ uint16_t float_to_bf16(float f) {
 
 int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void *in,
                 const void *w, const float *bias, void *out, int splits, float *partial,
-                int variant, hipStream_t s) {
+                int variant, hipStream_t s, const float *colscale, float oscale) {
     TDM_CHECK(d.C % 32 == 0 && d.N % 64 == 0, TDM_ERR_UNSUPPORTED,
               "convolution with %d->%d channels (need multiples of 32 and 64)", d.C, d.N);
     TDM_CHECK(d.ksize == 3 || d.ksize == 1, TDM_ERR_UNSUPPORTED, "window %d", d.ksize);
@@ -918,17 +984,25 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
     TDM_CHECK(splits >= 1 && splits <= 65535 && (splits == 1 || partial), TDM_ERR_INVALID_ARG,
               "bad split");
     if (batch <= 0) return TDM_OK;
-    if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
-    if (dt == DType::F16) {
-        if (out_f32) return conv_dispatch<f16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
-        return conv_dispatch<f16_t, f16_t>(d, batch, in, w, bias, out, splits, partial, variant, s);
+    if (dt == DType::F8) {
+        TDM_CHECK(colscale, TDM_ERR_INVALID_ARG, "fp8 convolution without scales");
+        if (out_f32)
+            return conv_dispatch<fp8_t, float>(d, batch, in, w, bias, out, splits, partial,
+                                               variant, s, colscale, oscale);
+        return conv_dispatch<fp8_t, fp8_t>(d, batch, in, w, bias, out, splits, partial, variant,
+                                           s, colscale, oscale);
     }
-    if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s);
-    return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, variant, s);
+    if (dt == DType::F32) return conv_dispatch<float, float>(d, batch, in, w, bias, out, splits, partial, variant, s, nullptr, 1.0f);
+    if (dt == DType::F16) {
+        if (out_f32) return conv_dispatch<f16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s, nullptr, 1.0f);
+        return conv_dispatch<f16_t, f16_t>(d, batch, in, w, bias, out, splits, partial, variant, s, nullptr, 1.0f);
+    }
+    if (out_f32) return conv_dispatch<bf16_t, float>(d, batch, in, w, bias, out, splits, partial, variant, s, nullptr, 1.0f);
+    return conv_dispatch<bf16_t, bf16_t>(d, batch, in, w, bias, out, splits, partial, variant, s, nullptr, 1.0f);
 }
 
 int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool relu, int batch,
-                 const void *w, const float *bias, void *out, hipStream_t s) {
+                 const void *w, const float *bias, void *out, hipStream_t s, float oscale) {
     TDM_CHECK(N % 64 == 0, TDM_ERR_UNSUPPORTED, "first convolution needs a multiple of 64 outputs");
     TDM_CHECK(src.P >= 3, TDM_ERR_INVALID_ARG, "patch extent %d", src.P);
     if (batch <= 0) return TDM_OK;
@@ -944,6 +1018,7 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
     a.out = out;
     a.N = N;
     a.relu = relu ? 1 : 0;
+    a.oscale = oscale;
     const long OD = src.P - 2;
     a.M = (long)batch * OD * OD * OD;
     dim3 grid((unsigned)((a.M + 255) / 256), N / 64);
@@ -957,6 +1032,8 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
         else conv1_kernel<T, OT, 0><<<grid, 256, 0, s>>>(a);                                  \
     } while (0)
     if (dt == DType::F32) TDM_C1(float, float);
+    else if (dt == DType::F8 && out_f32) TDM_C1(f16_t, float);
+    else if (dt == DType::F8) TDM_C1(f16_t, fp8_t);
     else if (dt == DType::F16 && out_f32) TDM_C1(f16_t, float);
     else if (dt == DType::F16) TDM_C1(f16_t, f16_t);
     else if (out_f32) TDM_C1(bf16_t, float);
@@ -967,16 +1044,55 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
 }
 
 int launch_to_ncdhw(DType dt, bool in_f32, const void *in, int batch, int C, int E, float *out,
-                    hipStream_t s) {
+                    hipStream_t s, float scale) {
     const long total = (long)batch * C * E * E * E;
     if (total <= 0) return TDM_OK;
     unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
     if (dt == DType::F32 || in_f32)
-        to_ncdhw_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E, out);
+        to_ncdhw_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float *>(in), batch, C, E, out, 1.0f);
+    else if (dt == DType::F8)
+        to_ncdhw_kernel<fp8_t><<<blocks, 256, 0, s>>>(static_cast<const fp8_t *>(in), batch, C, E, out, scale);
     else if (dt == DType::F16)
-        to_ncdhw_kernel<f16_t><<<blocks, 256, 0, s>>>(static_cast<const f16_t *>(in), batch, C, E, out);
+        to_ncdhw_kernel<f16_t><<<blocks, 256, 0, s>>>(static_cast<const f16_t *>(in), batch, C, E, out, 1.0f);
     else
-        to_ncdhw_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C, E, out);
+        to_ncdhw_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C, E, out, 1.0f);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+// Row r of `in` (per_row floats) times inv_scale[r], converted to e4m3.
+__global__ void quantize_fp8_kernel(const float *__restrict__ in, long rows, long per_row,
+                                    const float *__restrict__ inv_scale, fp8_t *__restrict__ out) {
+    const long total = rows * per_row;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = to_fp8(in[i] * inv_scale[i / per_row]);
+}
+
+__global__ void absmax_kernel(const float *__restrict__ x, long n, unsigned *__restrict__ amax) {
+    float m = 0.0f;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(x[i]));
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(m));
+}
+
+int launch_quantize_fp8(const float *in, long rows, long per_row, const float *inv_scale,
+                        void *out, hipStream_t s) {
+    const long total = rows * per_row;
+    if (total <= 0) return TDM_OK;
+    const unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
+    quantize_fp8_kernel<<<blocks, 256, 0, s>>>(in, rows, per_row, inv_scale,
+                                               static_cast<fp8_t *>(out));
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int launch_absmax(const float *x, long n, unsigned *amax, hipStream_t s) {
+    if (n <= 0) return TDM_OK;
+    const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 4096);
+    absmax_kernel<<<blocks, 256, 0, s>>>(x, n, amax);
     TDM_LAUNCHED();
     return TDM_OK;
 }
@@ -992,6 +1108,9 @@ int launch_maxpool(DType dt, const void *in, int batch, int C, int E, void *out,
     else if (dt == DType::F16)
         maxpool_kernel<f16_t><<<blocks, 256, 0, s>>>(static_cast<const f16_t *>(in), batch, C, E,
                                                      OE, static_cast<f16_t *>(out));
+    else if (dt == DType::F8)  // exact: the max of e4m3 values is one of them
+        maxpool_kernel<fp8_t><<<blocks, 256, 0, s>>>(static_cast<const fp8_t *>(in), batch, C, E,
+                                                     OE, static_cast<fp8_t *>(out));
     else
         maxpool_kernel<bf16_t><<<blocks, 256, 0, s>>>(static_cast<const bf16_t *>(in), batch, C,
                                                       E, OE, static_cast<bf16_t *>(out));

@@ -10,6 +10,7 @@
 // storage type of the precision (float or bf16); the last layer writes float
 // descriptors straight into the caller's buffer.
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -40,6 +41,12 @@ struct Layer {
     void *w_dev = nullptr;   // packed [N][taps][C] (conv1: [N][KP])
     float *b_dev = nullptr;
     int variant = -1;        // conv tile variant (-1: default; see tdm_net_autotune)
+    // fp8: per output channel weight scale (host), scale of this layer's
+    // output activations (set by tdm_net_calibrate), and the device
+    // dequantization scale per output channel (weight scale x input scale)
+    std::vector<float> wscale;
+    float act_scale = 0.0f;
+    float *colscale_dev = nullptr;
     int final_d() const { return pool ? pooled_d : out_d; }
     size_t out_elems() const { return (size_t)cout * final_d() * final_d() * final_d(); }
 };
@@ -219,13 +226,18 @@ int build_layers(const tdm::Json &root, tdm_net &net) {
 }
 
 // Packs host weights [O][I][k^3] into the device layout of the precision.
+// fp8: each output channel is scaled so its largest |w| maps to the largest
+// e4m3 value, quantized on the device, and the dequantization scale of the
+// channel (weight scale x the input activations' scale) goes to colscale.
 int upload_layer(tdm_net &net, Layer &l, size_t li) {
     const int taps = l.k * l.k * l.k;
-    const size_t esz = tdm::dtype_size(net.dt);
+    // the first convolution of an fp8 net computes in fp16
+    const tdm::DType wdt = (net.dt == tdm::DType::F8 && li == 0) ? tdm::DType::F16 : net.dt;
+    const size_t esz = tdm::dtype_size(wdt);
     std::vector<float> packed;
     int row = 0;
     if (li == 0) {
-        row = tdm::conv1_taps_padded(net.dt);
+        row = tdm::conv1_taps_padded(wdt);
         packed.assign((size_t)l.cout * row, 0.0f);
         for (int o = 0; o < l.cout; ++o)
             for (int t = 0; t < taps; ++t) packed[(size_t)o * row + t] = l.w[(size_t)o * taps + t];
@@ -241,11 +253,40 @@ int upload_layer(tdm_net &net, Layer &l, size_t li) {
     const size_t bytes = packed.size() * esz;
     if (!l.w_dev) TDM_HIP(hipMalloc(&l.w_dev, bytes));
     if (!l.b_dev) TDM_HIP(hipMalloc(&l.b_dev, sizeof(float) * l.cout));
-    if (net.dt == tdm::DType::F32) {
+    if (wdt == tdm::DType::F32) {
         TDM_HIP(hipMemcpy(l.w_dev, packed.data(), bytes, hipMemcpyHostToDevice));
+    } else if (wdt == tdm::DType::F8) {
+        l.wscale.assign(l.cout, 1.0f);
+        std::vector<float> inv(l.cout, 1.0f), cs(l.cout);
+        for (int o = 0; o < l.cout; ++o) {
+            float m = 0.0f;
+            for (int k = 0; k < row; ++k) m = std::max(m, std::fabs(packed[(size_t)o * row + k]));
+            if (m > 0.0f) {
+                l.wscale[o] = m / tdm::kFp8Max;
+                inv[o] = tdm::kFp8Max / m;
+            }
+            cs[o] = l.wscale[o] * net.layers[li - 1].act_scale;
+        }
+        float *tmp = nullptr, *dinv = nullptr;
+        TDM_HIP(hipMalloc(&tmp, packed.size() * sizeof(float)));
+        int st = TDM_OK;
+        if (hipMalloc(&dinv, sizeof(float) * l.cout) != hipSuccess) st = TDM_ERR_HIP;
+        if (st == TDM_OK &&
+            (hipMemcpy(tmp, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice) !=
+                 hipSuccess ||
+             hipMemcpy(dinv, inv.data(), sizeof(float) * l.cout, hipMemcpyHostToDevice) !=
+                 hipSuccess))
+            st = TDM_ERR_HIP;
+        if (st == TDM_OK) st = tdm::launch_quantize_fp8(tmp, l.cout, row, dinv, l.w_dev, nullptr);
+        if (st == TDM_OK && hipDeviceSynchronize() != hipSuccess) st = TDM_ERR_HIP;
+        (void)hipFree(tmp);
+        if (dinv) (void)hipFree(dinv);
+        if (st != TDM_OK) return tdm::fail(st, "%s: fp8 weight quantization failed", l.name.c_str());
+        if (!l.colscale_dev) TDM_HIP(hipMalloc(&l.colscale_dev, sizeof(float) * l.cout));
+        TDM_HIP(hipMemcpy(l.colscale_dev, cs.data(), sizeof(float) * l.cout, hipMemcpyHostToDevice));
     } else {
         std::vector<uint16_t> h(packed.size());
-        if (net.dt == tdm::DType::F16)
+        if (wdt == tdm::DType::F16)
             for (size_t i = 0; i < h.size(); ++i) h[i] = tdm::float_to_f16(packed[i]);
         else
             for (size_t i = 0; i < h.size(); ++i) h[i] = tdm::float_to_bf16(packed[i]);
@@ -294,6 +335,11 @@ int choose_splits(const tdm_net &net, const Layer &l, size_t nq_chunks) {
     return (int)std::max<long>(s, 1);
 }
 
+// Multiplier applied to a layer's outputs before fp8 conversion.
+float out_scale(const tdm_net &net, const Layer &l) {
+    return net.dt == tdm::DType::F8 && l.act_scale > 0.0f ? 1.0f / l.act_scale : 1.0f;
+}
+
 // Runs layers [0, upto] for `n` patches of one pass.  The input of the first
 // layer comes from `src`.  When upto is the last layer, its float output goes
 // to `final_out`; otherwise the output stays in *result (storage type).
@@ -330,11 +376,13 @@ int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *
         Layer &l = net.layers[li];
         const bool to_final = li == last && final_out;
         void *out = to_final ? static_cast<void *>(final_out) : net.act[1 - cur];
-        const bool out_f32 = to_final || net.dt == tdm::DType::F32;
+        // fp8 nets keep the last layer (the descriptor) in float
+        const bool out_f32 =
+            to_final || net.dt == tdm::DType::F32 || (net.dt == tdm::DType::F8 && li == last);
         if (li == 0) {
             void *dst = l.pool && !l.pool_fused ? net.act[1 - cur] : out;
             TDM_TRY(tdm::launch_conv1(net.dt, out_f32, src, l.cout, l.relu, n, l.w_dev, l.b_dev,
-                                      dst, s));
+                                      dst, s, out_scale(net, l)));
             cur = 1 - cur;
             if (l.pool) {
                 TDM_TRY(tdm::launch_maxpool(net.dt, net.act[cur], n, l.cout, l.out_d, net.act[1 - cur], s));
@@ -354,7 +402,8 @@ int run_layer(tdm_net &net, int li, const tdm::VolumeSource &src, int n, float *
         const int splits = choose_splits(net, l, nq);
         void *dst = (l.pool && !l.pool_fused) ? net.act[1 - cur] : out;
         TDM_TRY(tdm::launch_conv(net.dt, out_f32, d, n, net.act[cur], l.w_dev, l.b_dev, dst,
-                                 splits, net.partial, l.variant, s));
+                                 splits, net.partial, l.variant, s, l.colscale_dev,
+                                 out_scale(net, l)));
         cur = 1 - cur;
         if (l.pool && !l.pool_fused) {
             TDM_CHECK(!to_final, TDM_ERR_UNSUPPORTED, "pooling after the last layer");
@@ -402,6 +451,10 @@ int check_ready(tdm_net *net) {
     for (const Layer &l : net->layers)
         TDM_CHECK(l.has_w && l.has_b, TDM_ERR_INVALID_ARG, "layer %s has no weights loaded",
                   l.name.c_str());
+    if (net->dt == tdm::DType::F8)
+        for (size_t li = 0; li + 1 < net->layers.size(); ++li)
+            TDM_CHECK(net->layers[li].act_scale > 0.0f, TDM_ERR_INVALID_ARG,
+                      "fp8 net: activation scales not set (call tdm_net_calibrate)");
     TDM_TRY(bind_device(net));
     for (size_t li = 0; li < net->layers.size(); ++li)
         if (net->layers[li].dirty) {
@@ -437,7 +490,7 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
     tdm::clear_error();
     TDM_CHECK(json_path && out, TDM_ERR_INVALID_ARG, "null pointer argument");
     TDM_CHECK(precision == TDM_PREC_FP32 || precision == TDM_PREC_BF16 ||
-                  precision == TDM_PREC_FP16,
+                  precision == TDM_PREC_FP16 || precision == TDM_PREC_FP8,
               TDM_ERR_INVALID_ARG, "unknown precision %d", precision);
     TDM_CHECK(max_batch >= 0 && max_batch <= 65536, TDM_ERR_INVALID_ARG, "bad max_batch");
     std::string text, err;
@@ -449,6 +502,7 @@ int tdm_net_create(tdm_ctx *ctx, const char *json_path, int precision, int max_b
     net->device = ctx ? ctx->device : -1;
     net->dt = precision == TDM_PREC_FP32   ? tdm::DType::F32
               : precision == TDM_PREC_FP16 ? tdm::DType::F16
+              : precision == TDM_PREC_FP8  ? tdm::DType::F8
                                            : tdm::DType::BF16;
     net->max_batch = max_batch ? max_batch : 256;
     int s = build_layers(root, *net);
@@ -470,6 +524,7 @@ int tdm_net_destroy(tdm_net *net) {
     for (Layer &l : net->layers) {
         if (l.w_dev) (void)hipFree(l.w_dev);
         if (l.b_dev) (void)hipFree(l.b_dev);
+        if (l.colscale_dev) (void)hipFree(l.colscale_dev);
     }
     for (void *p : net->act)
         if (p) (void)hipFree(p);
@@ -667,12 +722,13 @@ int tdm_net_autotune(tdm_net *net, int num, void *stream) {
         const int splits = choose_splits(*net, l, nq);
         const bool out_f32 = net->dt == tdm::DType::F32 || li + 1 == net->layers.size();
         float best = 3.4e38f;
-        int best_v = tdm::conv_default_variant(d);
+        int best_v = tdm::conv_default_variant(net->dt, d);
         for (int v = 0; v < tdm::kConvVariants && st == TDM_OK; ++v) {
-            if (!tdm::conv_variant_valid(v, d)) continue;
+            if (!tdm::conv_variant_valid(net->dt, v, d)) continue;
             auto run = [&]() {
                 return tdm::launch_conv(net->dt, out_f32, d, n, net->act[0], l.w_dev, l.b_dev,
-                                        net->act[1], splits, net->partial, v, s);
+                                        net->act[1], splits, net->partial, v, s, l.colscale_dev,
+                                        out_scale(*net, l));
             };
             if ((st = run()) != TDM_OK) break;  // warm-up
             if (hipEventRecord(e0, s) != hipSuccess) st = TDM_ERR_HIP;
@@ -693,6 +749,66 @@ int tdm_net_autotune(tdm_net *net, int num, void *stream) {
     (void)hipEventDestroy(e1);
     if (st == TDM_ERR_HIP) return tdm::fail(st, "autotune timing failed");
     return st;
+}
+
+int tdm_net_calibrate(tdm_net *net, tdm_net *ref, const float *data, int num, void *stream) {
+    tdm::clear_error();
+    TDM_CHECK(net && ref && data && num >= 1, TDM_ERR_INVALID_ARG, "bad arguments");
+    TDM_CHECK(net->dt == tdm::DType::F8, TDM_ERR_INVALID_ARG, "calibration is for fp8 nets");
+    TDM_CHECK(ref->dt != tdm::DType::F8, TDM_ERR_INVALID_ARG, "the reference net must not be fp8");
+    TDM_CHECK(ref->layers.size() == net->layers.size(), TDM_ERR_INVALID_ARG,
+              "nets have different layers");
+    size_t most = 0;
+    for (size_t i = 0; i < net->layers.size(); ++i) {
+        const Layer &a = net->layers[i], &b = ref->layers[i];
+        TDM_CHECK(a.cin == b.cin && a.cout == b.cout && a.k == b.k && a.in_d == b.in_d &&
+                      a.pool == b.pool,
+                  TDM_ERR_INVALID_ARG, "layer %zu differs between the nets", i);
+        most = std::max(most, a.out_elems());
+    }
+    hipStream_t s = tdm::as_stream(stream);
+    float *tmp = nullptr;
+    unsigned *amax = nullptr;
+    TDM_HIP(hipMalloc(&tmp, most * num * sizeof(float)));
+    int st = TDM_OK;
+    if (hipMalloc(&amax, sizeof(unsigned)) != hipSuccess) st = TDM_ERR_HIP;
+    for (size_t li = 0; li + 1 < net->layers.size() && st == TDM_OK; ++li) {
+        const size_t count = net->layers[li].out_elems() * num;
+        st = tdm_net_forward_layer(ref, data, num, (int)li, tmp, count, stream);
+        if (st == TDM_OK && hipMemsetAsync(amax, 0, sizeof(unsigned), s) != hipSuccess)
+            st = TDM_ERR_HIP;
+        if (st == TDM_OK) st = tdm::launch_absmax(tmp, (long)count, amax, s);
+        unsigned bits = 0;
+        if (st == TDM_OK &&
+            (hipMemcpyAsync(&bits, amax, sizeof bits, hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess))
+            st = TDM_ERR_HIP;
+        float m;
+        std::memcpy(&m, &bits, sizeof m);
+        net->layers[li].act_scale = (st == TDM_OK && m > 0.0f) ? m / tdm::kFp8Max : 1.0f;
+    }
+    (void)hipFree(tmp);
+    if (amax) (void)hipFree(amax);
+    if (st != TDM_OK) return st == TDM_ERR_HIP ? tdm::fail(st, "calibration failed") : st;
+    for (Layer &l : net->layers) l.dirty = l.has_w;  // re-upload with the new scales
+    return TDM_OK;
+}
+
+int tdm_net_act_scale(const tdm_net *net, int index, float *scale) {
+    tdm::clear_error();
+    TDM_CHECK(net && scale && index >= 0 && index < (int)net->layers.size(), TDM_ERR_INVALID_ARG,
+              "bad arguments");
+    *scale = net->layers[index].act_scale;
+    return TDM_OK;
+}
+
+int tdm_net_set_act_scale(tdm_net *net, int index, float scale) {
+    tdm::clear_error();
+    TDM_CHECK(net && index >= 0 && index < (int)net->layers.size() && scale > 0.0f,
+              TDM_ERR_INVALID_ARG, "bad arguments");
+    net->layers[index].act_scale = scale;
+    for (Layer &l : net->layers) l.dirty = l.has_w;
+    return TDM_OK;
 }
 
 int tdm_net_layer_variant(const tdm_net *net, int index) {
@@ -729,8 +845,10 @@ int tdm_net_forward_layer(tdm_net *net, const float *data, int num, int layer_in
         src.P = net->patch;
         const void *res = nullptr;
         TDM_TRY(run_pass(*net, src, n, layer_index, nullptr, &res, s));
-        TDM_TRY(tdm::launch_to_ncdhw(net->dt, false, res, n, l.cout, l.final_d(),
-                                     out + (long)start * per, s));
+        const bool last_f32 =
+            net->dt == tdm::DType::F8 && layer_index + 1 == (int)net->layers.size();
+        TDM_TRY(tdm::launch_to_ncdhw(net->dt, last_f32, res, n, l.cout, l.final_d(),
+                                     out + (long)start * per, s, l.act_scale));
     }
     return TDM_OK;
 }

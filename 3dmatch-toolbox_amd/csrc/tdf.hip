@@ -410,6 +410,107 @@ __global__ void extract_patches_kernel(const float *__restrict__ tdf, int dx, in
 
 // ---- drivers -----------------------------------------------------------------------
 
+// ---- accurate TDF: distance from voxel centres to the nearest point ------------
+//
+// training/match.hpp:13-51 (Get1NNSearchDist) and
+// deprecated/pointCloud2AccTDF.m: tdf = 1 - min(d, trunc) / trunc with d the
+// Euclidean distance from the voxel centre to the nearest point.  Points are
+// binned into 8x8x8-voxel tiles (count, scan, fill); one workgroup per tile
+// stages the points of the tiles within reach through LDS and each thread
+// keeps the minimum squared distance of its voxel.
+constexpr int kAT = 8;  // tile edge (voxels)
+
+__device__ inline long acc_tile_of(int gx, int gy, int gz, int tx, int ty) {
+    return ((long)(gz / kAT) * ty + gy / kAT) * tx + gx / kAT;
+}
+
+__global__ void acc_count_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
+                                 float ox, float oy, float oz, float vs, int tx, int ty,
+                                 int *__restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int gx, gy, gz;
+    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
+    atomicAdd(&counts[acc_tile_of(gx, gy, gz, tx, ty)], 1);
+}
+
+// Exclusive scan of n counts (one workgroup); offsets[n] = total.
+__global__ __launch_bounds__(1024) void acc_scan_kernel(const int *__restrict__ counts, long n,
+                                                        int *__restrict__ offsets) {
+    __shared__ int part[1024];
+    const long per = (n + 1023) / 1024;
+    const long b0 = min(n, (long)threadIdx.x * per), b1 = min(n, b0 + per);
+    int sum = 0;
+    for (long b = b0; b < b1; ++b) sum += counts[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
+    for (long b = b0; b < b1; ++b) {
+        offsets[b] = run;
+        run += counts[b];
+    }
+    if (threadIdx.x == 1023) offsets[n] = part[1023];
+}
+
+__global__ void acc_fill_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
+                                float ox, float oy, float oz, float vs, int tx, int ty,
+                                const int *__restrict__ offsets, int *__restrict__ cursor,
+                                float4 *__restrict__ binned) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int gx, gy, gz;
+    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
+    const long t = acc_tile_of(gx, gy, gz, tx, ty);
+    const int slot = atomicAdd(&cursor[t], 1);
+    binned[offsets[t] + slot] = make_float4(pts[3L * i], pts[3L * i + 1], pts[3L * i + 2], 0.0f);
+}
+
+// One 8x8x8 tile per workgroup of 512 threads.  Points of the tiles within
+// `reach` tiles (reach*8 voxels >= the truncation radius plus the half voxel
+// a point may sit off its voxel) are staged through LDS 512 at a time.
+__global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
+                                                      const int *__restrict__ offsets, int dx,
+                                                      int dy, int dz, int tx, int ty, int tz,
+                                                      int reach, float ox, float oy, float oz,
+                                                      float vs, float trunc,
+                                                      float *__restrict__ tdf) {
+    __shared__ float4 sp[512];
+    const int t = blockIdx.x;
+    const int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
+    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+    const int x = bx * kAT + lx, y = by * kAT + ly, z = bz * kAT + lz;
+    // voxel centre (origin is the centre of voxel 0)
+    const float cx = ox + (float)x * vs, cy = oy + (float)y * vs, cz = oz + (float)z * vs;
+    float best = __builtin_huge_valf();
+    for (int nz = max(0, bz - reach); nz <= min(tz - 1, bz + reach); ++nz)
+        for (int ny = max(0, by - reach); ny <= min(ty - 1, by + reach); ++ny)
+            for (int nx = max(0, bx - reach); nx <= min(tx - 1, bx + reach); ++nx) {
+                const long nt = ((long)nz * ty + ny) * tx + nx;
+                const int p0 = offsets[nt], p1 = offsets[nt + 1];
+                for (int c0 = p0; c0 < p1; c0 += 512) {
+                    const int cn = min(512, p1 - c0);
+                    __syncthreads();
+                    if ((int)threadIdx.x < cn) sp[threadIdx.x] = binned[c0 + threadIdx.x];
+                    __syncthreads();
+                    for (int k = 0; k < cn; ++k) {
+                        const float4 q = sp[k];
+                        const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
+                        const float d2 = ex * ex + ey * ey + ez * ez;
+                        best = fminf(best, d2);
+                    }
+                }
+            }
+    if (x >= dx || y >= dy || z >= dz) return;
+    const float d = sqrtf(best);  // sqrt is monotone: the minimum of the distances
+    tdf[((long)z * dy + y) * dx + x] = 1.0f - fminf(d, trunc) / trunc;
+}
+
 int check_dims(int dx, int dy, int dz) {
     TDM_CHECK(dx > 0 && dy > 0 && dz > 0, TDM_ERR_INVALID_ARG, "bad grid %dx%dx%d", dx, dy, dz);
     TDM_CHECK((double)dx * dy * dz <= 2147483647.0, TDM_ERR_INVALID_ARG,
@@ -614,6 +715,60 @@ int tdm_extract_patches(tdm_ctx *ctx, const float *tdf, int dim_x, int dim_y, in
     unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 65536);
     extract_patches_kernel<<<blocks, 256, 0, tdm::as_stream(stream)>>>(
         tdf, dim_x, dim_y, dim_z, kp_grid, num, patch_size, patches);
+    TDM_LAUNCHED();
+    return TDM_OK;
+}
+
+int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                     const float origin[3], float voxel_size, float trunc_margin, float *tdf,
+                     void *stream) {
+    tdm::clear_error();
+    TDM_CHECK((pts || num_pts == 0) && dims && origin && tdf, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_CHECK(num_pts >= 0, TDM_ERR_INVALID_ARG, "negative point count");
+    TDM_CHECK(voxel_size > 0 && trunc_margin > 0, TDM_ERR_INVALID_ARG, "bad voxel size / margin");
+    TDM_TRY(check_dims(dims[0], dims[1], dims[2]));
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    hipStream_t s = tdm::as_stream(stream);
+    const int tx = (dims[0] + kAT - 1) / kAT, ty = (dims[1] + kAT - 1) / kAT,
+              tz = (dims[2] + kAT - 1) / kAT;
+    const long nt = (long)tx * ty * tz;
+    // a point within trunc of a voxel centre is binned at most
+    // trunc/vs + 1/2 voxels away from that voxel
+    const double reach_vox = (double)trunc_margin / voxel_size + 0.5;
+    const int reach = (int)std::ceil(reach_vox / kAT);
+    TDM_CHECK(reach <= 4, TDM_ERR_UNSUPPORTED, "truncation of %g voxels is too wide",
+              (double)trunc_margin / voxel_size);
+    const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1) + 16 + sizeof(float4) * (size_t)num_pts;
+    void *scratch = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &scratch, s));
+    int *counts = static_cast<int *>(scratch);
+    int *offsets = counts + nt;  // nt + 1 entries
+    float4 *binned = reinterpret_cast<float4 *>(
+        (reinterpret_cast<uintptr_t>(offsets + nt + 1) + 15) & ~uintptr_t(15));
+    const unsigned fb = (unsigned)std::min<long>((nt + 255) / 256, 16384);
+    fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
+    TDM_LAUNCHED();
+    if (num_pts > 0) {
+        acc_count_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            tx, ty, counts);
+        TDM_LAUNCHED();
+    }
+    acc_scan_kernel<<<1, 1024, 0, s>>>(counts, nt, offsets);
+    TDM_LAUNCHED();
+    fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
+    TDM_LAUNCHED();
+    if (num_pts > 0) {
+        acc_fill_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            tx, ty, offsets, counts, binned);
+        TDM_LAUNCHED();
+    }
+    TDM_CHECK(nt < (1L << 31), TDM_ERR_INVALID_ARG, "grid too large");
+    acc_tdf_kernel<<<(unsigned)nt, 512, 0, s>>>(binned, offsets, dims[0], dims[1], dims[2], tx, ty,
+                                                tz, reach, origin[0], origin[1], origin[2],
+                                                voxel_size, trunc_margin, tdf);
     TDM_LAUNCHED();
     return TDM_OK;
 }

@@ -32,6 +32,7 @@ ERR_NO_DEVICE = 7
 PREC_FP32 = 0
 PREC_BF16 = 1
 PREC_FP16 = 2
+PREC_FP8 = 3
 DECODE_EXACT = 0
 DECODE_REFERENCE = 1
 
@@ -178,6 +179,13 @@ def tdf_from_points(pts, dims, origin, voxel_size, trunc, tdf, decode=DECODE_EXA
     _check(lib().tdm_tdf_from_points(None, _p(pts), c_int(pts.shape[0]), _int3(dims),
                                      _float3(origin), _f(voxel_size), _f(trunc), c_int(decode),
                                      _p(tdf), _stream(stream)))
+
+
+def tdf_accurate(pts, dims, origin, voxel_size, trunc, tdf, stream=None) -> None:
+    """Voxel-centre to nearest-point TDF (training/match.hpp:13-51)."""
+    _check(lib().tdm_tdf_accurate(None, _p(pts), c_int(pts.shape[0]), _int3(dims),
+                                  _float3(origin), _f(voxel_size), _f(trunc), _p(tdf),
+                                  _stream(stream)))
 
 
 class _CloudRef(ctypes.Structure):
@@ -433,6 +441,21 @@ class Net:
         _check(lib().tdm_net_forward_grid(self._h, _p(tdf), c_int(dims[0]), c_int(dims[1]),
                                           c_int(dims[2]), _p(kp_grid), c_int(kp_grid.shape[0]),
                                           _p(feat), _stream(stream)))
+
+    def calibrate(self, ref: "Net", data, stream=None) -> list[float]:
+        """fp8 nets: per-layer activation scales from `ref` (a float/16-bit
+        net with the same weights) on patches `data` [n, 30, 30, 30]."""
+        _check(lib().tdm_net_calibrate(self._h, ref._h, _p(data), c_int(data.shape[0]),
+                                       _stream(stream)))
+        return [self.act_scale(i) for i in range(len(self.layer_names))]
+
+    def act_scale(self, index: int) -> float:
+        v = c_float()
+        _check(lib().tdm_net_act_scale(self._h, c_int(index), ctypes.byref(v)))
+        return float(v.value)
+
+    def set_act_scale(self, index: int, scale: float) -> None:
+        _check(lib().tdm_net_set_act_scale(self._h, c_int(index), _f(scale)))
 
     def autotune(self, num: int = 0, stream=None) -> list[int]:
         """Times each convolution's tile variants and keeps the fastest;

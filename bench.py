@@ -65,6 +65,8 @@ class DescriptorPipeline:
         self.keypts = keypts
         self.net = tdm.Net(js, precision, max_batch=batch)
         self.net.load_weights(wt)
+        if precision == tdm.PREC_FP8:
+            calibrate_fp8(torch, self.net, js, wt, self.pts, keypts[:256])
         # tile autotuner: bit-identical results, fastest tile per layer
         self.variants = self.net.autotune(batch)
         self.vs = 0.01
@@ -98,6 +100,26 @@ def layer_tflops(net):
         flops = 2 * od ** 3 * cout * cin * k ** 3
         out[name] = flops * patches / (ms / 1e3) / 1e12 if ms > 0 else None
     return out
+
+
+def calibrate_fp8(torch, net, js, wt, pts, keypts):
+    """Activation scales of an fp8 net from an fp32 net on the patches around
+    `keypts` of the cloud `pts` (device)."""
+    import numpy as np
+
+    import tdm
+
+    dims, origin = tdm.grid_params(pts, np.float32(0.01), 15)
+    tdf = torch.empty(dims[2], dims[1], dims[0], device=pts.device)
+    tdm.tdf_from_points(pts, dims, origin, np.float32(0.01), np.float32(0.05), tdf)
+    kg = torch.from_numpy(tdm.keypoints_to_grid(keypts, origin, np.float32(0.01))).to(pts.device)
+    patches = torch.empty(len(kg), 30, 30, 30, device=pts.device)
+    tdm.extract_patches(tdf, dims, kg, patches)
+    ref = tdm.Net(js, tdm.PREC_FP32, max_batch=256)
+    ref.load_weights(wt)
+    scales = net.calibrate(ref, patches)
+    ref.close()
+    return scales
 
 
 def time_steps(torch, fn, steps, warmup):
@@ -243,9 +265,13 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
 
     dev = torch.device("cuda:0")
     nets = {p: tdm.Net(js, p, max_batch=1024)
-            for p in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16)}
+            for p in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8)}
     for n in nets.values():
         n.load_weights(wt)
+    # fp8 activation scales from the fp32 net on 256 patches of fragment 0
+    c0 = synth.fragment(200_000, view=0, seed=10)
+    calibrate_fp8(torch, nets[tdm.PREC_FP8], js, wt, torch.from_numpy(c0).to(dev),
+                  c0[synth.keypoint_indices(len(c0), 256, seed=999)])
     desc = {p: [] for p in nets}
     first_patches = None
     for f in range(n_frag):
@@ -269,7 +295,7 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
     d32 = torch.cat(desc[tdm.PREC_FP32])
     ab32, m32 = mutual_nn(torch, desc[tdm.PREC_FP32][0], desc[tdm.PREC_FP32][1])
     vs_fp32 = {}
-    for p, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+    for p, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16"), (tdm.PREC_FP8, "fp8")):
         d16 = torch.cat(desc[p])
         ab16, m16 = mutual_nn(torch, desc[p][0], desc[p][1])
         both = m32 & m16 & (ab32 == ab16)
@@ -283,7 +309,8 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
     p_host = first_patches.cpu().numpy()
     ref = oracle.net_forward_fp64(p_host, dict(weights), shapes)
     per_layer = {}
-    for p, name in ((tdm.PREC_FP32, "fp32"), (tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+    for p, name in ((tdm.PREC_FP32, "fp32"), (tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16"),
+                    (tdm.PREC_FP8, "fp8")):
         errs = []
         for i, (_n, _ci, co, _k, _d, od, pooled) in enumerate(shapes):
             e = pooled if pooled is not None else od
@@ -699,7 +726,8 @@ def main() -> int:
     torch.cuda.empty_cache()
 
     if not args.no_extras:
-        for prec, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16")):
+        for prec, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16"),
+                           (tdm.PREC_FP8, "fp8")):
             log(f"{name} descriptor pipeline")
             pb = DescriptorPipeline(torch, cloud, keypts, js, wt, prec, args.batch)
             pb.net.set_profiling(True)
@@ -711,7 +739,7 @@ def main() -> int:
                 "ms_per_step": sb * 1e3,
                 "speedup_vs_fp32": (n_kp / sb) / value,
                 "conv2_tflops": tf,
-                "conv2_frac_of_16bit_peak": tf / MFMA_PEAK_TFLOPS["bf16"],
+                "conv2_frac_of_peak": tf / MFMA_PEAK_TFLOPS[name],
                 "tile_variants": pb.variants,
                 "layer_tflops": layer_tflops(pb.net),
             }

@@ -50,6 +50,11 @@ enum tdm_precision {
     TDM_PREC_FP16 = 2, /* fp16 weights/activations, fp32 accumulation: the half
                           storage / float compute of the reference's default
                           build (DATATYPE 0, core/marvin.hpp:11-27)          */
+    TDM_PREC_FP8 = 3,  /* e4m3 weights (one scale per output channel) and
+                          activations (one scale per layer, from
+                          tdm_net_calibrate), fp32 accumulation; the first
+                          convolution computes in fp16, the last layer's
+                          output (the descriptor) stays float               */
 };
 
 /* How ComputeTDF turns a linear voxel index into (x, y, z). */
@@ -164,6 +169,16 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
                         const float origin[3], float voxel_size, float trunc_margin, int decode,
                         float *tdf, void *stream);
 
+/* Accurate TDF (training/match.hpp:13-51 Get1NNSearchDist,
+ * deprecated/pointCloud2AccTDF.m): tdf = 1 - min(d, trunc_margin) /
+ * trunc_margin with d the Euclidean distance from each voxel centre
+ * (origin + index * voxel_size) to the nearest point, in float: d^2 summed
+ * as (dx^2 + dy^2) + dz^2, sqrtf correctly rounded.  Points outside the
+ * grid are ignored.  pts, tdf: device.  Reach up to 31.5 voxels. */
+int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
+                     const float origin[3], float voxel_size, float trunc_margin, float *tdf,
+                     void *stream);
+
 /* Many clouds in one launch per step: the TDF of core/demo.cu:127-158 for a
  * batch of fragments of different sizes, as a plan (made once, on the host)
  * and runs (three launches for the whole batch: clear the bit grids, scatter
@@ -243,6 +258,18 @@ int tdm_net_forward(tdm_net *net, const float *data, int num, float *feat, void 
  * host gather and the per-batch copies).  tdf, kp_grid, feat: device. */
 int tdm_net_forward_grid(tdm_net *net, const float *tdf, int dim_x, int dim_y, int dim_z,
                          const int *kp_grid, int num, float *feat, void *stream);
+
+/* Activation scales of an fp8 net (no counterpart in the reference, which
+ * has no 8-bit type): runs `ref` (an fp32/fp16/bf16 net with the same layers
+ * and weights) on `num` patches (device, [num,1,30,30,30]) and sets each
+ * layer's output scale to max|activation| / 448, so the calibration set's
+ * largest activation maps to the largest e4m3 value.  An fp8 net refuses to
+ * run until its scales are set.  Synchronizes `stream`. */
+int tdm_net_calibrate(tdm_net *net, tdm_net *ref, const float *data, int num, void *stream);
+/* Read or set one layer's output activation scale directly (e.g. to reuse a
+ * calibration). */
+int tdm_net_act_scale(const tdm_net *net, int index, float *scale);
+int tdm_net_set_act_scale(tdm_net *net, int index, float scale);
 
 /* Tile autotuner (no counterpart in the reference, whose convolutions are
  * cuDNN's, core/marvin.hpp:4635-4709): times every tile variant of each

@@ -251,3 +251,38 @@ def surface_points(tsdf: np.ndarray, weight: np.ndarray, origin, voxel_size, tsd
     vs = np.float32(voxel_size)
     return np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
                      o[2] + z.astype(np.float32) * vs], axis=1)
+
+
+def tdf_accurate(pts, dims, origin, voxel_size, trunc) -> np.ndarray:
+    """Get1NNSearchDist (training/match.hpp:13-51) on the demo grid: for each
+    voxel centre origin + idx*vs (float32), d = sqrtf((dx^2 + dy^2) + dz^2)
+    to the nearest point in float32, tdf = 1 - min(d, trunc)/trunc.  Points
+    outside the grid (by the occupancy rounding) are ignored.  The candidates
+    of each voxel come from a float64 KD-tree query with a margin; the
+    distances themselves are evaluated in float32 in the reference's order."""
+    from scipy.spatial import cKDTree
+
+    pts = np.asarray(pts, np.float32)
+    dx, dy, dz = dims
+    o = np.asarray(origin, np.float32)
+    vs, tr = np.float32(voxel_size), np.float32(trunc)
+    g = np.round((pts - o) / vs)
+    inside = np.all((g >= 0) & (g < np.array([dx, dy, dz])), axis=1)
+    pts = pts[inside]
+    z, y, x = np.meshgrid(np.arange(dz), np.arange(dy), np.arange(dx), indexing="ij")
+    c = np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
+                  o[2] + z.astype(np.float32) * vs], -1).reshape(-1, 3)
+    out = np.zeros(len(c), np.float32)
+    if len(pts) == 0:
+        return out.reshape(dz, dy, dx)
+    tree = cKDTree(pts.astype(np.float64))
+    cand = tree.query_ball_point(c.astype(np.float64), float(tr) * 1.0001 + 1e-6)
+    for i, lst in enumerate(cand):
+        if not lst:
+            continue
+        q = pts[lst]
+        e = c[i] - q
+        d2 = (e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]
+        d = np.sqrt(d2.min())
+        out[i] = np.float32(1.0) - np.minimum(d, tr) / tr
+    return out.reshape(dz, dy, dx)

@@ -4,7 +4,10 @@ against the FP64 oracle (torch CPU float64 conv3d/relu/max_pool3d).
 Tolerances (relative L2 over each layer's whole output, NCDHW):
   fp32 path  <= 1e-5 per layer and for the descriptors;
   bf16 path  <= 3e-2 per layer, <= 3e-2 for the descriptors;
-  fp16 path  <= 1e-2 per layer and for the descriptors.
+  fp16 path  <= 1e-2 per layer and for the descriptors;
+  fp8 path   <= 1e-1 per layer and for the descriptors (e4m3 with per-channel
+             weight scales and per-layer activation scales calibrated from the
+             fp32 net on the same patches).
 """
 import os
 
@@ -21,6 +24,7 @@ pytestmark = pytest.mark.gpu
 FP32_TOL = 1e-5
 BF16_TOL = 3e-2
 FP16_TOL = 1e-2
+FP8_TOL = 1e-1  # e4m3: 3 mantissa bits, per-channel weight / per-layer activation scales
 
 
 @pytest.fixture(scope="module")
@@ -69,14 +73,26 @@ def _run_layers(torch, net, p):
     return outs
 
 
-PRECISIONS = [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL), (tdm.PREC_FP16, FP16_TOL)]
+PRECISIONS = [(tdm.PREC_FP32, FP32_TOL), (tdm.PREC_BF16, BF16_TOL), (tdm.PREC_FP16, FP16_TOL),
+              (tdm.PREC_FP8, FP8_TOL)]
+
+
+def _make_net(torch, js, wt, prec, calib, max_batch=0):
+    """A net of `prec`; fp8 nets are calibrated from an fp32 net on `calib`."""
+    net = tdm.Net(js, prec, max_batch=max_batch)
+    net.load_weights(wt)
+    if prec == tdm.PREC_FP8:
+        ref = tdm.Net(js, tdm.PREC_FP32, max_batch=max_batch)
+        ref.load_weights(wt)
+        net.calibrate(ref, torch.from_numpy(np.ascontiguousarray(calib)).to("cuda:0"))
+        ref.close()
+    return net
 
 
 @pytest.mark.parametrize("prec,tol", PRECISIONS)
 def test_per_layer_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
     js, wt, _ = net_files
-    net = tdm.Net(js, prec, max_batch=4)
-    net.load_weights(wt)
+    net = _make_net(torch_gpu, js, wt, prec, patches[0], max_batch=4)
     outs = _run_layers(torch_gpu, net, patches[0])
     errs = [rel_l2(o, r) for o, r in zip(outs, fp64)]
     print("per-layer rel-L2:", ["%.2e" % e for e in errs], flush=True)
@@ -87,8 +103,7 @@ def test_per_layer_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
 @pytest.mark.parametrize("prec,tol", PRECISIONS)
 def test_descriptors_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
     js, wt, _ = net_files
-    net = tdm.Net(js, prec)
-    net.load_weights(wt)
+    net = _make_net(torch_gpu, js, wt, prec, patches[0])
     p = patches[0]
     data = torch_gpu.from_numpy(p).to("cuda:0")
     feat = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
@@ -103,9 +118,8 @@ def test_grid_path_equals_patch_path(torch_gpu, net_files, patches):
     """Patch gather fused into conv1 gives bit-identical descriptors."""
     js, wt, _ = net_files
     p, tdf, dims, kg = patches
-    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
-        net = tdm.Net(js, prec)
-        net.load_weights(wt)
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8):
+        net = _make_net(torch_gpu, js, wt, prec, p)
         a = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
         b = torch_gpu.empty_like(a)
         net.forward(torch_gpu.from_numpy(p).to("cuda:0"), a)
@@ -147,7 +161,7 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
     torch = torch_gpu
     js, wt, _ = net_files
     vs = np.float32(0.01)
-    grids, kps = [], []
+    grids, kps, host_tdfs = [], [], []
     for k, (n, view) in enumerate([(30_000, 0), (45_000, 10), (20_000, 25)]):
         cloud = synth.fragment(n, view=view, seed=40 + k)
         dims, origin = oracle.grid_params(cloud, vs, 15)
@@ -157,13 +171,14 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
         kg = oracle.keypoint_grid(cloud[idx], origin, vs)
         grids.append((torch.from_numpy(tdf).to("cuda:0"), dims))
         kps.append(kg)
+        host_tdfs.append(tdf)
     order = np.concatenate([np.full(len(k), i) for i, k in enumerate(kps)])
     kp_all = np.concatenate(kps)
     perm = np.random.default_rng(3).permutation(len(order))
     table = tdm.grid_table(grids)
-    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
-        net = tdm.Net(js, prec, max_batch=8)
-        net.load_weights(wt)
+    calib = oracle.extract_patches(host_tdfs[0], kps[0][:4])
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8):
+        net = _make_net(torch, js, wt, prec, calib, max_batch=8)
         per = []
         for (t, dims), kg in zip(grids, kps):
             f = torch.empty(len(kg), 512, dtype=torch.float32, device="cuda:0")
@@ -182,9 +197,8 @@ def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches)
     torch = torch_gpu
     js, wt, _ = net_files
     p = torch.from_numpy(patches[0]).to("cuda:0")
-    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
-        net = tdm.Net(js, prec, max_batch=4)
-        net.load_weights(wt)
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8):
+        net = _make_net(torch, js, wt, prec, patches[0], max_batch=4)
         ref = torch.empty(len(p), 512, dtype=torch.float32, device="cuda:0")
         net.forward(p, ref)
         for v in range(9):

@@ -135,3 +135,22 @@ def test_extract_patches_bit_exact(torch_gpu, cloud):
     out = torch_gpu.empty(len(kg), 30, 30, 30, dtype=torch_gpu.float32, device="cuda:0")
     tdm.extract_patches(_dev(torch_gpu, tdf), dims, _dev(torch_gpu, kg), out)
     assert np.array_equal(out.cpu().numpy(), oracle.extract_patches(tdf, kg))
+
+
+def test_accurate_tdf_bit_exact(torch_gpu):
+    """tdm_tdf_accurate equals the oracle's voxel-centre nearest-point TDF
+    bit for bit (float32, the reference's summation order, correctly rounded
+    sqrt) on a synthetic fragment grid."""
+    torch = torch_gpu
+    cloud = synth.fragment(8_000, view=3, seed=77)
+    vs, tr = np.float32(0.01), np.float32(0.05)
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    dims = [min(d, 90) for d in dims]  # a sub-block keeps the oracle fast; outside points drop
+    want = oracle.tdf_accurate(cloud, dims, origin, vs, tr)
+    pts = torch.from_numpy(cloud).to("cuda:0")
+    got = torch.full((dims[2], dims[1], dims[0]), -1.0, device="cuda:0")
+    tdm.tdf_accurate(pts, dims, origin, vs, tr, got)
+    torch.cuda.synchronize()
+    g = got.cpu().numpy()
+    assert np.count_nonzero(want) > 1000
+    assert np.array_equal(g.view(np.uint32), want.view(np.uint32)), np.abs(g - want).max()

@@ -115,3 +115,30 @@ def test_patch_gather_zero_padded():
     for k, p in zip(kg, got):
         x, y, z = k + 30
         assert np.array_equal(p, pad[z - 15:z + 15, y - 15:y + 15, x - 15:x + 15])
+
+
+def test_accurate_tdf_oracle_vs_brute_force():
+    """The KD-tree-assisted accurate TDF oracle against a plain brute force
+    over every point (float32, the reference's summation order) on a tiny
+    grid, and basic closed forms: a voxel holding a point at its centre reads
+    1, voxels farther than the margin from every point read 0."""
+    rng = np.random.default_rng(12)
+    dims = (14, 11, 9)
+    origin = np.array([0.1, -0.2, 0.3], np.float32)
+    vs, tr = np.float32(0.01), np.float32(0.05)
+    pts = (origin + rng.uniform(0, 1, (40, 3)) * np.array(dims) * vs).astype(np.float32)
+    pts[0] = origin + np.float32(3) * vs  # exactly on the centre of voxel (3, 3, 3)
+    got = oracle.tdf_accurate(pts, dims, origin, vs, tr)
+    z, y, x = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+    c = np.stack([origin[0] + x.astype(np.float32) * vs, origin[1] + y.astype(np.float32) * vs,
+                  origin[2] + z.astype(np.float32) * vs], -1).reshape(-1, 3)
+    g = np.round((pts - origin) / vs)
+    keep = np.all((g >= 0) & (g < np.array(dims)), axis=1)
+    e = c[:, None, :] - pts[None, keep, :]
+    d2 = (e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]
+    d = np.sqrt(d2.min(1))
+    want = (np.float32(1) - np.minimum(d, tr) / tr).reshape(dims[2], dims[1], dims[0])
+    assert np.array_equal(got, want)
+    assert got[3, 3, 3] == 1.0
+    far = d > tr
+    assert np.all(got.reshape(-1)[far] == 0.0)
