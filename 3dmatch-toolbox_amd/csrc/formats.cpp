@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pipeline.hpp"
 #include "marvin_io.hpp"
 
 #pragma clang fp contract(off)
@@ -180,6 +181,12 @@ const CofTerms kCof[16] = {
 const signed char kCofPattern[6] = {+1, -1, -1, +1, +1, -1};
 
 }  // namespace
+
+namespace tdm {
+int read_png_u16(const char *path, std::vector<uint16_t> &pix, int &H, int &W) {
+    return decode_png_u16(path, pix, H, W);
+}
+}  // namespace tdm
 
 extern "C" {
 

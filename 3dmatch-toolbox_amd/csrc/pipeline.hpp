@@ -4,6 +4,8 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
+#include <vector>
 
 namespace tdm {
 
@@ -16,5 +18,8 @@ int grid_from_box(const float box[6], float voxel_size, int padding, int dims[3]
 size_t match_scratch_bytes(int num_a, int num_b, int dim);
 // Scratch (context slot 3) used by tdm_ransac_rt for max_trials.
 size_t ransac_scratch_bytes(long max_trials);
+
+// Decodes a 16-bit grayscale PNG (depth-fusion/utils.hpp:74-86's reader).
+int read_png_u16(const char *path, std::vector<uint16_t> &pix, int &H, int &W);
 
 }  // namespace tdm

@@ -148,6 +148,40 @@ def surface_points(tsdf, weight, dims, origin, voxel_size, pts=None, capacity=0,
     return int(n.value)
 
 
+class Fusion:
+    """tdm_fusion: depth PNG frames fused with overlapped decode/copy/integrate."""
+
+    def __init__(self, cam_K, height, width, dims, origin, voxel_size, trunc, threads=0):
+        K = (c_float * 9)(*[float(v) for v in np.asarray(cam_K, np.float32).reshape(9)])
+        self._h = c_void_p()
+        _check(lib().tdm_fusion_create(None, K, c_int(height), c_int(width), _int3(dims),
+                                       _float3(origin), _f(voxel_size), _f(trunc), c_int(threads),
+                                       ctypes.byref(self._h)))
+        self.dims = list(dims)
+
+    def add_png(self, path: str, cam2base) -> None:
+        P = (c_float * 16)(*[float(v) for v in np.asarray(cam2base, np.float32).reshape(16)])
+        _check(lib().tdm_fusion_add_png(self._h, path.encode(), P))
+
+    def wait(self) -> None:
+        _check(lib().tdm_fusion_wait(self._h))
+
+    def copy_grids(self, tsdf, weight, stream=None) -> None:
+        """Waits, then copies the fused grids into device tensors."""
+        _check(lib().tdm_fusion_copy_grids(self._h, _p(tsdf), _p(weight), _stream(stream)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().tdm_fusion_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def depth_from_mm(mm, out, stream=None) -> None:
     _check(lib().tdm_depth_from_mm(None, _p(mm), c_size_t(mm.numel()), _p(out), _stream(stream)))
 

@@ -135,6 +135,28 @@ int tdm_surface_points(tdm_ctx *ctx, const float *tsdf, const float *weight, con
                        float weight_thresh, float *pts, size_t capacity, size_t *num_points,
                        void *stream);
 
+/* Frame fusion with overlapped I/O (the frame loop of
+ * depth-fusion/demo.cu:156-183 as a pipeline): decoder threads read the
+ * 16-bit depth PNGs into a ring of pinned buffers while earlier frames are
+ * copied (as u16), converted to metres on the GPU (depth-fusion/utils.hpp:
+ * 80-84) and integrated, in submission order on the fusion's own stream, so
+ * the grids are bit-identical to the serial loop.  The fusion owns a
+ * TSDF/weight grid pair initialised as tdm_tsdf_init does.  threads: decoder
+ * threads (0 = 4).  cam_K, cam2base, dims, origin: host. */
+typedef struct tdm_fusion tdm_fusion;
+int tdm_fusion_create(tdm_ctx *ctx, const float cam_K[9], int im_height, int im_width,
+                      const int dims[3], const float origin[3], float voxel_size,
+                      float trunc_margin, int threads, tdm_fusion **out);
+/* Queues one frame: the PNG path and its camera-to-base pose (copied). */
+int tdm_fusion_add_png(tdm_fusion *f, const char *depth_png, const float cam2base[16]);
+/* Waits until every queued frame is integrated; reports the first error. */
+int tdm_fusion_wait(tdm_fusion *f);
+/* The fusion's device grids (valid until tdm_fusion_destroy). */
+int tdm_fusion_grids(tdm_fusion *f, float **tsdf, float **weight);
+/* Waits, then copies the grids into caller buffers (device) on `stream`. */
+int tdm_fusion_copy_grids(tdm_fusion *f, float *tsdf, float *weight, void *stream);
+int tdm_fusion_destroy(tdm_fusion *f);
+
 /* u16 millimetre depth (device, n values) -> metres with values above 6 m set
  * to 0 (depth-fusion/utils.hpp:74-86 after the PNG decode). */
 int tdm_depth_from_mm(tdm_ctx *ctx, const uint16_t *depth_mm, size_t n, float *depth_m,
