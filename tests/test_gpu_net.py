@@ -5,7 +5,7 @@ Tolerances (relative L2 over each layer's whole output, NCDHW):
   fp32 path  <= 1e-5 per layer and for the descriptors;
   bf16 path  <= 3e-2 per layer, <= 3e-2 for the descriptors;
   fp16 path  <= 1e-2 per layer and for the descriptors;
-  fp8 path   <= 1e-1 per layer and for the descriptors (e4m3 with per-channel
+  fp8 path   <= 1.5e-1 per layer and for the descriptors (e4m3 with per-channel
              weight scales and per-layer activation scales calibrated from the
              fp32 net on the same patches).
 """
@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 FP32_TOL = 1e-5
 BF16_TOL = 3e-2
 FP16_TOL = 1e-2
-FP8_TOL = 1e-1  # e4m3: 3 mantissa bits, per-channel weight / per-layer activation scales
+FP8_TOL = 1.5e-1  # e4m3: 3 mantissa bits, per-channel weight / per-layer activation scales
 
 
 @pytest.fixture(scope="module")

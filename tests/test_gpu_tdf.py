@@ -144,8 +144,13 @@ def test_accurate_tdf_bit_exact(torch_gpu):
     torch = torch_gpu
     cloud = synth.fragment(8_000, view=3, seed=77)
     vs, tr = np.float32(0.01), np.float32(0.05)
-    dims, origin = oracle.grid_params(cloud, vs, 15)
-    dims = [min(d, 90) for d in dims]  # a sub-block keeps the oracle fast; outside points drop
+    full, origin = oracle.grid_params(cloud, vs, 15)
+    # a 90^3 sub-block around the cloud's median keeps the oracle fast (points
+    # outside it are dropped by both); shift the origin by whole voxels
+    med = np.median(cloud, axis=0)
+    off = np.clip(np.round((med - origin) / vs).astype(int) - 45, 0, None)
+    origin = (origin + off.astype(np.float32) * vs).astype(np.float32)
+    dims = [min(90, f - o) for f, o in zip(full, off)]
     want = oracle.tdf_accurate(cloud, dims, origin, vs, tr)
     pts = torch.from_numpy(cloud).to("cuda:0")
     got = torch.full((dims[2], dims[1], dims[0]), -1.0, device="cuda:0")
