@@ -91,6 +91,19 @@ class DescriptorPipeline:
         self.net.forward_grid(self.tdf, dims, self.kp_dev, self.feat)
 
 
+def measured_traffic(precision):
+    """conv2's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None if there is none for the
+    precision)."""
+    import re
+
+    path = os.path.join(REPO, "profiles", f"hbm_traffic_descriptor_pipeline_{precision}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"conv2 \(`[^`]*`\): (\d+) MB read and (\d+) MB written", open(path).read())
+    return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
+
+
 def layer_tflops(net):
     from tdm import netdef
 
@@ -715,7 +728,8 @@ def main() -> int:
             "peak": MFMA_PEAK_TFLOPS[args.precision],
             "unit": "TFLOP/s",
             "frac": achieved / MFMA_PEAK_TFLOPS[args.precision],
-            "traffic": None,
+            "traffic": measured_traffic(args.precision),
+            "traffic_source": "profiles/hbm_traffic_descriptor_pipeline_<precision>.md (1000-patch launch)",
             "flop_per_launch": conv2_flops * (patches2 / max(passes2, 1)),
             "avg_launch_ms": ms2 / max(passes2, 1),
         },
