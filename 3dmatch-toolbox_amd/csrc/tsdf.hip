@@ -2,26 +2,37 @@
 //
 // The reference kernel (depth-fusion/demo.cu:18-68) runs one thread per
 // (z, y) column with a serial loop over x, so the 64 lanes of a wave touch
-// addresses dim_x*4 bytes apart.  Here a lane owns one (x, y) and sweeps a
-// run of z: the lanes of a wave cover consecutive x, every grid access is a
-// coalesced 256-byte row segment, and the per-column part of the camera
-// transform is computed once per lane instead of once per voxel.
+// addresses dim_x*4 bytes apart.  Here a wave owns a tile of 64 consecutive x
+// of one row y and a run of kZRun slices z: every grid access is a coalesced
+// 256-byte row segment, the per-column part of the camera transform is
+// computed once per lane, and the per-slice part once per tile.
+//
+// Work per voxel is cut three ways, none of which changes a result:
+//  - a tile whose voxels all project outside the image (or behind the
+//    camera) is rejected from its four corners before any voxel is touched;
+//  - the pixel index comes from a reciprocal instead of two divisions when
+//    the pixel coordinate is far enough from a rounding boundary for the
+//    reference's rounding to agree (certified below); the rest, and every
+//    voxel whose distance needs them, use the reference's own divisions;
+//  - voxels in front of the truncation band skip the square root and the
+//    divisions whose outcome is already fixed.
 //
 // Bit-exactness: this file is compiled with -ffp-contract=off and IEEE
-// division/sqrt, and every per-voxel expression keeps the reference's
-// operation order, so pixel indices, the skip tests and the weights are
-// bit-identical and the TSDF values equal the reference arithmetic.
-// powf(v, 2) of the reference is evaluated as v*v (the single correctly
-// rounded product that gcc and clang both fold it to).
+// division/sqrt, and every value that reaches the grid (camera coordinates,
+// pixel index, distance, running mean) is computed with the reference's
+// operations in the reference's order, so pixel indices, the skip tests and
+// the weights are bit-identical and the TSDF values equal the reference
+// arithmetic.  powf(v, 2) of the reference is evaluated as v*v (the single
+// correctly rounded product that gcc and clang both fold it to).
 #include "common.hpp"
 
 #pragma clang fp contract(off)
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kZRun = 16;  // z slices swept by one lane (one frame)
-constexpr int kZRunF = 8;  // z slices kept in registers across frames
+constexpr int kRows = 4;   // rows (one per wave) of a workgroup
+constexpr int kZRun = 8;   // z slices of a tile
+constexpr float kU = 0x1p-24f;  // unit roundoff of float
 
 // Per-frame camera constants as the kernel uses them.
 struct Cam {
@@ -41,7 +52,8 @@ __device__ inline Cam load_cam(const float *__restrict__ K, const float *__restr
 }
 
 // Column constants of one (x, y) for one frame: the first two terms of each
-// row of R^T (p - t), summed left to right as the reference does.
+// row of R^T (p - t), summed left to right as the reference does
+// (depth-fusion/demo.cu:35-41).
 struct ColPart {
     float sx, sy, sz;
 };
@@ -56,62 +68,140 @@ __device__ inline ColPart col_part(const Cam &c, float bx, float by) {
     return p;
 }
 
-// The per-voxel body of depth-fusion/demo.cu:34-58 for one frame, split into
-// project() (camera coordinates, pixel) and finish() (depth tests, distance).
-//
-// Shortcuts skip work whose outcome is already decided, without changing any
-// result:
-//  - a voxel more than one pixel outside the image is rejected with products
-//    instead of the two divisions (the margin dwarfs the rounding error of
-//    the products; voxels near the border take the exact path);
-//  - with e = d - pcz, the reference's diff = e*s has s = sqrtf(1+qx^2+qy^2)
-//    >= 1, so e <= -trunc implies diff <= -trunc (skip), and e >= trunc
-//    implies diff/trunc >= 1, i.e. dist = 1; the square root and the
-//    division are only evaluated between the two.
-struct Proj {
-    int pix = -1;  // depth-image index, or -1 when the voxel is not seen
-    float pcz = 0.0f, qx = 0.0f, qy = 0.0f;
-};
-
-// depth-fusion/demo.cu:34-49: camera coordinates and pixel of a voxel.
-__device__ inline Proj project(const Cam &c, const ColPart &p, float bz, int H, int W) {
-    Proj r;
-    float t2 = bz - c.tz;
-    float pcx = p.sx + c.r20 * t2;
-    float pcy = p.sy + c.r21 * t2;
-    float pcz = p.sz + c.r22 * t2;
-    if (!(pcz > 0.0f)) return r;
-    // fx*pcx/pcz + cx < -1.5, > W + 0.5 (and the same for y): certainly outside.
-    const float lx = c.fx * pcx, ly = c.fy * pcy;
-    if (lx + (c.cx + 1.5f) * pcz < 0.0f || lx + (c.cx - ((float)W + 0.5f)) * pcz > 0.0f ||
-        ly + (c.cy + 1.5f) * pcz < 0.0f || ly + (c.cy - ((float)H + 0.5f)) * pcz > 0.0f)
-        return r;
-    const float qx = pcx / pcz;
-    const float qy = pcy / pcz;
-    // roundf = half away from zero; the range test runs on the rounded float,
-    // which rejects exactly what the reference's int conversion + test rejects.
-    const float rx = roundf(c.fx * qx + c.cx);
-    const float ry = roundf(c.fy * qy + c.cy);
-    if (!(rx >= 0.0f && rx < (float)W && ry >= 0.0f && ry < (float)H)) return r;
-    r.pix = (int)ry * W + (int)rx;
-    r.qx = qx;
-    r.qy = qy;
-    r.pcz = pcz;
-    return r;
+// Third term of each row for slice z (uniform over a tile's lanes):
+// pt_cam = ColPart + SliceTerm, as (a + b) + c of the reference.
+__device__ inline float4 slice_term(const Cam &c, float oz, float vs, int z) {
+    const float t2 = (oz + z * vs) - c.tz;
+    return make_float4(c.r20 * t2, c.r21 * t2, c.r22 * t2, 0.0f);
 }
 
-// depth-fusion/demo.cu:51-62 given the depth sample d of a seen voxel.
-__device__ inline bool finish(const Proj &r, float d, float trunc, float &dist) {
-    if (d <= 0.0f || d > 6.0f) return false;
-    const float e = d - r.pcz;
-    if (e <= -trunc) return false;
-    if (e >= trunc) {
-        dist = 1.0f;
-        return true;
+// ---- Bounds that make the shortcuts exact ----
+//
+// Pixel coordinate.  The reference computes q = fl(pc_x / pc_z),
+// v = fl(fl(fx * q) + cx) and the pixel roundf(v).  Here
+// va = fma(fx, fl(pc_x * rcp(pc_z)), cx), where v_rcp_f32 errs by at most
+// one ulp (2u relative; u = 2^-24).  Then |fl(pc_x * rcp) - q| <= 4u|Q|
+// (Q = pc_x / pc_z exactly) and |va - v| <= 5u|fx Q| + 2u|v| to first order.
+// Only va in [-1, W] matters (elsewhere both sides reject the voxel), where
+// |v| <= W + 1 and |fx Q| <= W + 1 + |cx|, so
+//   B = u (7 (W + 1 + |cx|) + 2 (W + 1)) + 2^-20
+// bounds |va - v| with room for a 2-ulp reciprocal.  If |va - rint(va)| <
+// 0.5 - B, v lies in the same open rounding interval as va, so
+// roundf(v) = rint(va).  Otherwise (about 1 voxel in 2000) the reference's
+// own division decides.
+//
+// Tile rejection.  With S_a = |o_a| + dim_a vs + |t_a|, the float camera
+// coordinates of any voxel differ from the exact affine function of its
+// index, R^T(o + i vs - t), by at most eta = 8u max_r (sum_a |R_ar| S_a)
+// (three roundings in the differences, two in the products and sums).  A
+// linear form of the exact coordinates that is negative at a tile's four
+// corners is negative on the whole tile.  The image-edge forms
+// f pc + (c + 1.5) pc_z (and the three others) put the edge one pixel
+// outside the reference's rejection boundary -0.5 (resp. W - 0.5); where
+// pc_z >= zeta the float pixel coordinate is within half a pixel of the
+// exact one, so a tile with all corners beyond a margin that covers the
+// float evaluation of the form is rejected by the reference voxel by voxel.
+struct FrameBounds {
+    float hx, hy;          // certified when |va - rint(va)| < h
+    float eta, zeta;
+    float mL, mR, mT, mB;  // margins of the four edge forms
+    float Wf, Hf;
+    bool edges;            // fx > 0 and fy > 0 (the edge forms assume it)
+};
+
+__device__ inline FrameBounds frame_bounds(const Cam &c, int H, int W, int dx, int dy, int dz,
+                                           float ox, float oy, float oz, float vs) {
+    FrameBounds f;
+    const float Wf = (float)W, Hf = (float)H;
+    const float acx = fabsf(c.cx), acy = fabsf(c.cy);
+    // each bound is rounded up by a factor covering its own float evaluation
+    f.hx = 0.5f - (kU * (7.0f * (Wf + 1.0f + acx) + 2.0f * (Wf + 1.0f)) + 0x1p-20f) * 1.01f;
+    f.hy = 0.5f - (kU * (7.0f * (Hf + 1.0f + acy) + 2.0f * (Hf + 1.0f)) + 0x1p-20f) * 1.01f;
+    const float Sx = fabsf(ox) + (float)dx * vs + fabsf(c.tx);
+    const float Sy = fabsf(oy) + (float)dy * vs + fabsf(c.ty);
+    const float Sz = fabsf(oz) + (float)dz * vs + fabsf(c.tz);
+    const float ex = fabsf(c.r00) * Sx + fabsf(c.r10) * Sy + fabsf(c.r20) * Sz;
+    const float ey = fabsf(c.r01) * Sx + fabsf(c.r11) * Sy + fabsf(c.r21) * Sz;
+    const float ez = fabsf(c.r02) * Sx + fabsf(c.r12) * Sy + fabsf(c.r22) * Sz;
+    const float pmax = fmaxf(ex, fmaxf(ey, ez)) * 1.01f;  // bounds |camera coordinate|
+    const float eta = 8.0f * kU * pmax + 1e-30f;
+    f.eta = eta * 1.01f;
+    f.zeta = 1.02f * fmaxf(4.0f * eta, 8.0f * eta * (fmaxf(c.fx, c.fy) + fmaxf(Wf, Hf) +
+                                                    fmaxf(acx, acy) + 1.0f));
+    const float cL = c.cx + 1.5f, cR = Wf + 0.5f - c.cx, cT = c.cy + 1.5f, cB = Hf + 0.5f - c.cy;
+    const float g = 1.03f * (eta + 4.0f * kU * pmax);
+    f.mL = (c.fx + fabsf(cL)) * g + 1e-6f;
+    f.mR = (c.fx + fabsf(cR)) * g + 1e-6f;
+    f.mT = (c.fy + fabsf(cT)) * g + 1e-6f;
+    f.mB = (c.fy + fabsf(cB)) * g + 1e-6f;
+    f.Wf = Wf;
+    f.Hf = Hf;
+    f.edges = c.fx > 0.0f && c.fy > 0.0f;
+    return f;
+}
+
+// True when no voxel of the tile (x in [bx0, bx1] as world coordinates, row
+// by, slice terms z0 / z1 of its first and last slice) can be seen.  Lanes
+// take the corners (lane & 1: x end, lane & 2: z end); every lane of the wave
+// must call it.
+__device__ inline bool tile_hidden(const Cam &c, const FrameBounds &f, float bx0, float bx1,
+                                   float by, float4 z0, float4 z1) {
+    const int lane = threadIdx.x & 63;
+    const ColPart q = col_part(c, (lane & 1) ? bx1 : bx0, by);
+    const float4 t = (lane & 2) ? z1 : z0;
+    const float px = q.sx + t.x, py = q.sy + t.y, pz = q.sz + t.z;
+    if (__all(pz <= -2.5f * f.eta)) return true;  // behind the camera
+    if (!f.edges || !__all(pz >= f.zeta + 1.5f * f.eta)) return false;
+    const float cL = c.cx + 1.5f, cR = f.Wf + 0.5f - c.cx, cT = c.cy + 1.5f, cB = f.Hf + 0.5f - c.cy;
+    const float hL = c.fx * px + cL * pz, hR = cR * pz - c.fx * px;
+    const float hT = c.fy * py + cT * pz, hB = cB * pz - c.fy * py;
+    return __all(hL < -f.mL) || __all(hR < -f.mR) || __all(hT < -f.mT) || __all(hB < -f.mB);
+}
+
+// roundf(f * (pc / pz) + c) of depth-fusion/demo.cu:46-47 with the range test
+// of :48-49; -1 when outside [0, n).
+__device__ inline int exact_index(float pc, float pz, float f, float c, int n) {
+    const float q = pc / pz;
+    const float r = roundf(f * q + c);
+    return (r >= 0.0f && r < (float)n) ? (int)r : -1;
+}
+
+// Depth-image index of a voxel with camera coordinates (pcx, pcy, pcz), or
+// -1 when the reference skips it at depth-fusion/demo.cu:42-49.
+__device__ inline int pixel_of(const Cam &c, const FrameBounds &f, float pcx, float pcy,
+                               float pcz, int H, int W) {
+    const float r = __builtin_amdgcn_rcpf(pcz);
+    const float vx = fmaf(c.fx, pcx * r, c.cx);
+    const float vy = fmaf(c.fy, pcy * r, c.cy);
+    const float rx = rintf(vx), ry = rintf(vy);
+    const bool cert = pcz >= 0x1p-64f && fabsf(vx - rx) < f.hx && fabsf(vy - ry) < f.hy;
+    const bool in = rx >= 0.0f && rx < f.Wf && ry >= 0.0f && ry < f.Hf;
+    int pix = (cert && in) ? (int)fmaf(ry, f.Wf, rx) : -1;  // exact: integers below 2^24
+    if (!cert && pcz > 0.0f) {
+        const int ix = exact_index(pcx, pcz, c.fx, c.cx, W);
+        const int iy = exact_index(pcy, pcz, c.fy, c.cy, H);
+        pix = (ix >= 0 && iy >= 0) ? iy * W + ix : -1;
     }
-    float diff = e * sqrtf(1.0f + r.qx * r.qx + r.qy * r.qy);
-    if (diff <= -trunc) return false;
-    dist = fminf(1.0f, diff / trunc);
+    return pix;
+}
+
+// depth-fusion/demo.cu:51-62 for a seen voxel with depth sample d.  With
+// e = d - pcz, the reference's diff = e*s has s = sqrtf(1+qx^2+qy^2) >= 1, so
+// e <= -trunc implies diff <= -trunc (skip) and e >= trunc implies
+// diff/trunc >= 1, i.e. dist = 1; the divisions and the square root are only
+// evaluated between the two.
+__device__ inline bool tsdf_dist(float d, float pcx, float pcy, float pcz, float trunc,
+                                 float &dist) {
+    if (d <= 0.0f || d > 6.0f) return false;
+    const float e = d - pcz;
+    if (e <= -trunc) return false;
+    dist = 1.0f;
+    if (e < trunc) {
+        const float qx = pcx / pcz, qy = pcy / pcz;
+        const float diff = e * sqrtf(1.0f + qx * qx + qy * qy);
+        if (diff <= -trunc) return false;
+        dist = fminf(1.0f, diff / trunc);
+    }
     return true;
 }
 
@@ -123,97 +213,163 @@ __device__ inline float running_mean(float t, float w, float w_new, float dist) 
     return (t * w + dist) / w_new;
 }
 
-// One frame; lane = (x, y), blockIdx.y = run of kZRun z slices.
-__global__ __launch_bounds__(kThreads) void integrate_frame_kernel(
+// Grid element at a byte offset from a (uniform) slice base: the address is a
+// scalar base plus a 32-bit lane offset.
+__device__ inline float &at_byte(float *base, unsigned byte_off) {
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off);
+}
+__device__ inline float at_byte(const float *base, unsigned byte_off) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// One frame.  Workgroup = kRows waves; wave = row y = blockIdx.y*kRows + wave,
+// lanes = x in [blockIdx.x*64, +64), slices z in [blockIdx.z*kZRun, +kZRun).
+__global__ __launch_bounds__(64 * kRows) void integrate_frame_kernel(
     const float *__restrict__ K, const float *__restrict__ cam2base,
     const float *__restrict__ depth, int H, int W, int dx, int dy, int dz, float ox, float oy,
     float oz, float vs, float trunc, float *__restrict__ tsdf, float *__restrict__ weight) {
-    const long plane = (long)dx * dy;
-    const long col = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (col >= plane) return;
-    const int x = (int)(col % dx);
-    const int y = (int)(col / dx);
+    __shared__ float4 slices[kRows][kZRun];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int y = blockIdx.y * kRows + wave;
+    if (y >= dy) return;  // whole wave
     const Cam c = load_cam(K, cam2base);
-    const ColPart p = col_part(c, ox + x * vs, oy + y * vs);
-    const int z0 = blockIdx.y * kZRun;
-    const int z1 = min(dz, z0 + kZRun);
-    for (int z = z0; z < z1; ++z) {
-        const Proj r = project(c, p, oz + z * vs, H, W);
-        if (r.pix < 0) continue;
-        float dist;
-        if (!finish(r, depth[r.pix], trunc, dist)) continue;
-        const long i = z * plane + col;
-        const float w_old = weight[i];
-        const float w_new = w_old + 1.0f;
-        const float t_old = tsdf[i];
-        const float t_new = running_mean(t_old, w_old, w_new, dist);
-        weight[i] = w_new;
-        // an unchanged value (same bits) needs no store
-        if (__float_as_uint(t_new) != __float_as_uint(t_old)) tsdf[i] = t_new;
+    const int z0 = blockIdx.z * kZRun;
+    const int nz = min(kZRun, dz - z0);
+    const int x0 = blockIdx.x * 64;
+    const int x = x0 + lane;
+    float4 *zt = slices[wave];
+    const float4 mine = slice_term(c, oz, vs, z0 + min(lane, nz - 1));
+    if (lane < kZRun) zt[lane] = mine;
+    const FrameBounds f = frame_bounds(c, H, W, dx, dy, dz, ox, oy, oz, vs);
+    const float by = oy + y * vs;
+    float4 first, last;
+    first.x = __shfl(mine.x, 0); first.y = __shfl(mine.y, 0); first.z = __shfl(mine.z, 0);
+    last.x = __shfl(mine.x, nz - 1); last.y = __shfl(mine.y, nz - 1); last.z = __shfl(mine.z, nz - 1);
+    if (tile_hidden(c, f, ox + x0 * vs, ox + min(x0 + 63, dx - 1) * vs, by, first, last)) return;
+    if (x >= dx) return;
+    const ColPart p = col_part(c, ox + x * vs, by);
+    const unsigned plane = (unsigned)dx * (unsigned)dy;
+    const unsigned boff = ((unsigned)y * (unsigned)dx + (unsigned)x) * 4u;
+    int pix[kZRun];
+    float pz[kZRun];
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j) {
+        const float4 t = zt[j];
+        pz[j] = p.sz + t.z;
+        const int v = pixel_of(c, f, p.sx + t.x, p.sy + t.y, pz[j], H, W);
+        pix[j] = j < nz ? v : -1;
     }
+    float d[kZRun];
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j) d[j] = at_byte(depth, (unsigned)max(pix[j], 0) * 4u);
+    unsigned upd = 0;
+    float dist[kZRun];
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j) {
+        const float4 t = zt[j];
+        if (pix[j] >= 0 && tsdf_dist(d[j], p.sx + t.x, p.sy + t.y, pz[j], trunc, dist[j]))
+            upd |= 1u << j;
+    }
+    float w[kZRun], tt[kZRun];
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j)
+        if (upd & (1u << j)) {
+            const size_t zoff = (size_t)(z0 + j) * plane;
+            w[j] = at_byte(weight + zoff, boff);
+            tt[j] = at_byte(tsdf + zoff, boff);
+        }
+#pragma unroll
+    for (int j = 0; j < kZRun; ++j)
+        if (upd & (1u << j)) {
+            const size_t zoff = (size_t)(z0 + j) * plane;
+            const float w_new = w[j] + 1.0f;
+            const float t_new = running_mean(tt[j], w[j], w_new, dist[j]);
+            at_byte(weight + zoff, boff) = w_new;
+            // an unchanged value (same bits) needs no store
+            if (__float_as_uint(t_new) != __float_as_uint(tt[j])) at_byte(tsdf + zoff, boff) = t_new;
+        }
 }
 
-// All frames in one pass: a lane keeps the kZRunF voxels of its run in
-// registers, applies the frames in order, reads a voxel the first time it is
-// updated and writes back only updated voxels.  Per voxel the sequence of
-// updates is the reference's, so the result is bit-identical.
-__global__ __launch_bounds__(kThreads) void integrate_frames_kernel(
+// All frames in one pass: a wave keeps its tile's voxels in registers,
+// applies the frames in order, reads a voxel the first time it is updated and
+// writes back only updated voxels.  Per voxel the sequence of updates is the
+// reference's, so the result is bit-identical.  Scene blockIdx.x / ntx of a
+// batch (ntx = x tiles per row) uses its own poses, frames, grid and origin.
+__global__ __launch_bounds__(64 * kRows) void integrate_frames_kernel(
     const float *__restrict__ K, const float *__restrict__ cam2base,
     const float *__restrict__ depths, int F, int H, int W, int dx, int dy, int dz, float ox,
     float oy, float oz, float vs, float trunc, float *__restrict__ tsdf,
-    float *__restrict__ weight, const float *__restrict__ origins) {
-    const long plane = (long)dx * dy;
-    const long col = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (col >= plane) return;
-    // scene blockIdx.z of a batch: its own poses, frames, grid and origin
-    if (gridDim.z > 1 || origins) {
-        const long sc = blockIdx.z;
-        cam2base += sc * F * 16;
-        depths += sc * F * (long)H * W;
-        tsdf += sc * plane * dz;
-        weight += sc * plane * dz;
-        if (origins) {
-            ox = origins[3 * sc + 0];
-            oy = origins[3 * sc + 1];
-            oz = origins[3 * sc + 2];
-        }
+    float *__restrict__ weight, const float *__restrict__ origins, int ntx) {
+    __shared__ float4 slices[kRows][kZRun];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sc = blockIdx.x / ntx;
+    const int y = blockIdx.y * kRows + wave;
+    if (y >= dy) return;  // whole wave
+    const unsigned plane = (unsigned)dx * (unsigned)dy;
+    if (origins) {
+        cam2base += (size_t)sc * F * 16;
+        depths += (size_t)sc * F * H * W;
+        tsdf += (size_t)sc * plane * dz;
+        weight += (size_t)sc * plane * dz;
+        ox = origins[3 * sc + 0];
+        oy = origins[3 * sc + 1];
+        oz = origins[3 * sc + 2];
     }
-    const int x = (int)(col % dx);
-    const int y = (int)(col / dx);
-    const float bx = ox + x * vs;
-    const float by = oy + y * vs;
-    const int z0 = blockIdx.y * kZRunF;
-    const int nz = min(dz - z0, kZRunF);
-    const long base = z0 * plane + col;
-    float t[kZRunF], w[kZRunF];
+    const int z0 = blockIdx.z * kZRun;
+    const int nz = min(kZRun, dz - z0);
+    const int x0 = (blockIdx.x - sc * ntx) * 64;
+    const int x = x0 + lane;
+    const bool live = x < dx;
+    const float bx = ox + x * vs, by = oy + y * vs;
+    const float bx0 = ox + x0 * vs, bx1 = ox + min(x0 + 63, dx - 1) * vs;
+    const unsigned boff = ((unsigned)y * (unsigned)dx + (unsigned)min(x, dx - 1)) * 4u;
+    const size_t frame_px = (size_t)H * W;
+    float4 *zt = slices[wave];
+    float t[kZRun], w[kZRun];
     unsigned loaded = 0;
-    const long frame_px = (long)H * W;
-    for (int f = 0; f < F; ++f) {
-        const Cam c = load_cam(K, cam2base + f * 16);
+    for (int fr = 0; fr < F; ++fr) {
+        const Cam c = load_cam(K, cam2base + fr * 16);
+        const float4 mine = slice_term(c, oz, vs, z0 + min(lane, nz - 1));
+        // the previous frame's reads of the table are done (same wave, in order)
+        if (lane < kZRun) zt[lane] = mine;
+        const FrameBounds f = frame_bounds(c, H, W, dx, dy, dz, ox, oy, oz, vs);
+        float4 first, last;
+        first.x = __shfl(mine.x, 0); first.y = __shfl(mine.y, 0); first.z = __shfl(mine.z, 0);
+        last.x = __shfl(mine.x, nz - 1); last.y = __shfl(mine.y, nz - 1); last.z = __shfl(mine.z, nz - 1);
+        if (tile_hidden(c, f, bx0, bx1, by, first, last) || !live) continue;
         const ColPart p = col_part(c, bx, by);
-        const float *__restrict__ depth = depths + f * frame_px;
-        Proj pr[kZRunF];
+        const float *__restrict__ depth = depths + fr * frame_px;
+        int pix[kZRun];
+        float pz[kZRun];
 #pragma unroll
-        for (int j = 0; j < kZRunF; ++j)
-            pr[j] = j < nz ? project(c, p, oz + (z0 + j) * vs, H, W) : Proj{};
-        float d[kZRunF];
+        for (int j = 0; j < kZRun; ++j) {
+            const float4 q = zt[j];
+            pz[j] = p.sz + q.z;
+            const int v = pixel_of(c, f, p.sx + q.x, p.sy + q.y, pz[j], H, W);
+            pix[j] = j < nz ? v : -1;
+        }
+        float d[kZRun];
 #pragma unroll
-        for (int j = 0; j < kZRunF; ++j) d[j] = depth[max(pr[j].pix, 0)];
+        for (int j = 0; j < kZRun; ++j) d[j] = at_byte(depth, (unsigned)max(pix[j], 0) * 4u);
         unsigned upd = 0;
-        float dist[kZRunF];
+        float dist[kZRun];
 #pragma unroll
-        for (int j = 0; j < kZRunF; ++j)
-            if (pr[j].pix >= 0 && finish(pr[j], d[j], trunc, dist[j])) upd |= 1u << j;
+        for (int j = 0; j < kZRun; ++j) {
+            const float4 q = zt[j];
+            if (pix[j] >= 0 && tsdf_dist(d[j], p.sx + q.x, p.sy + q.y, pz[j], trunc, dist[j]))
+                upd |= 1u << j;
+        }
         const unsigned fresh = upd & ~loaded;
 #pragma unroll
-        for (int j = 0; j < kZRunF; ++j)
+        for (int j = 0; j < kZRun; ++j)
             if (fresh & (1u << j)) {
-                t[j] = tsdf[base + j * plane];
-                w[j] = weight[base + j * plane];
+                const size_t zoff = (size_t)(z0 + j) * plane;
+                t[j] = at_byte(tsdf + zoff, boff);
+                w[j] = at_byte(weight + zoff, boff);
             }
         loaded |= upd;
 #pragma unroll
-        for (int j = 0; j < kZRunF; ++j)
+        for (int j = 0; j < kZRun; ++j)
             if (upd & (1u << j)) {
                 const float w_new = w[j] + 1.0f;
                 t[j] = running_mean(t[j], w[j], w_new, dist[j]);
@@ -221,12 +377,12 @@ __global__ __launch_bounds__(kThreads) void integrate_frames_kernel(
             }
     }
 #pragma unroll
-    for (int j = 0; j < kZRunF; ++j) {
+    for (int j = 0; j < kZRun; ++j)
         if (loaded & (1u << j)) {
-            tsdf[base + j * plane] = t[j];
-            weight[base + j * plane] = w[j];
+            const size_t zoff = (size_t)(z0 + j) * plane;
+            at_byte(tsdf + zoff, boff) = t[j];
+            at_byte(weight + zoff, boff) = w[j];
         }
-    }
 }
 
 __global__ void tsdf_init_kernel(float *__restrict__ tsdf, float *__restrict__ weight, size_t n) {
@@ -253,6 +409,21 @@ int check_grid(int H, int W, int dx, int dy, int dz, float vs, float trunc) {
     TDM_CHECK((double)dx * dy * dz < 9.0e18, TDM_ERR_INVALID_ARG, "grid too large");
     TDM_CHECK((long)H * W < (1L << 31), TDM_ERR_INVALID_ARG, "image too large");
     TDM_CHECK(vs > 0 && trunc > 0, TDM_ERR_INVALID_ARG, "voxel size and truncation must be > 0");
+    // the kernels address a slice and the depth image with 32-bit byte offsets
+    TDM_CHECK((double)dx * dy * 4.0 < 4294967296.0, TDM_ERR_INVALID_ARG,
+              "grid slice too large (%d x %d)", dx, dy);
+    TDM_CHECK((double)H * W * 4.0 < 4294967296.0 && W < (1 << 24) && H < (1 << 24),
+              TDM_ERR_INVALID_ARG, "image too large");
+    return TDM_OK;
+}
+
+// Launch grid of the tile kernels: x = 64-voxel tiles of a row (times the
+// scene count), y = groups of kRows rows, z = runs of kZRun slices.
+int tile_grid(int dx, int dy, int dz, int scenes, dim3 *grid) {
+    const long gx = (long)((dx + 63) / 64) * scenes;
+    const long gy = (dy + kRows - 1) / kRows, gz = (dz + kZRun - 1) / kZRun;
+    TDM_CHECK(gx < (1L << 31) && gy < 65536 && gz < 65536, TDM_ERR_INVALID_ARG, "grid too large");
+    *grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)gz);
     return TDM_OK;
 }
 
@@ -372,10 +543,9 @@ int tdm_integrate(tdm_ctx *ctx, const float *cam_K, const float *cam2base, const
               "null pointer argument");
     TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
-    long plane = (long)dim_x * dim_y;
-    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRun - 1) / kZRun);
-    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
-    integrate_frame_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+    dim3 grid;
+    TDM_TRY(tile_grid(dim_x, dim_y, dim_z, 1, &grid));
+    integrate_frame_kernel<<<grid, 64 * kRows, 0, tdm::as_stream(stream)>>>(
         cam_K, cam2base, depth_im, im_height, im_width, dim_x, dim_y, dim_z, origin_x, origin_y,
         origin_z, voxel_size, trunc_margin, tsdf, weight);
     TDM_LAUNCHED();
@@ -394,12 +564,12 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     if (num_frames == 0) return TDM_OK;
-    long plane = (long)dim_x * dim_y;
-    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRunF - 1) / kZRunF);
-    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
-    integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+    dim3 grid;
+    TDM_TRY(tile_grid(dim_x, dim_y, dim_z, 1, &grid));
+    integrate_frames_kernel<<<grid, 64 * kRows, 0, tdm::as_stream(stream)>>>(
         cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z,
-        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight, nullptr);
+        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight, nullptr,
+        (int)grid.x);
     TDM_LAUNCHED();
     return TDM_OK;
 }
@@ -417,12 +587,11 @@ int tdm_integrate_scenes(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     TDM_TRY(check_grid(im_height, im_width, dim_x, dim_y, dim_z, voxel_size, trunc_margin));
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     if (num_frames == 0 || num_scenes == 0) return TDM_OK;
-    long plane = (long)dim_x * dim_y;
-    dim3 grid(tdm::blocks_for(plane, kThreads), (dim_z + kZRunF - 1) / kZRunF, num_scenes);
-    TDM_CHECK(grid.x < (1u << 31) && grid.y < 65536, TDM_ERR_INVALID_ARG, "grid too large");
-    integrate_frames_kernel<<<grid, kThreads, 0, tdm::as_stream(stream)>>>(
+    dim3 grid;
+    TDM_TRY(tile_grid(dim_x, dim_y, dim_z, num_scenes, &grid));
+    integrate_frames_kernel<<<grid, 64 * kRows, 0, tdm::as_stream(stream)>>>(
         cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z, 0.0f,
-        0.0f, 0.0f, voxel_size, trunc_margin, tsdf, weight, origins);
+        0.0f, 0.0f, voxel_size, trunc_margin, tsdf, weight, origins, (int)(grid.x / num_scenes));
     TDM_LAUNCHED();
     return TDM_OK;
 }
