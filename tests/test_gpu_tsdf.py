@@ -67,6 +67,37 @@ def test_synthetic_frames_bit_exact(torch_gpu, synth_frames, fused):
     _assert_same(got, ref)
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_rounding_boundaries_bit_exact(torch_gpu, fused):
+    """Voxels that project onto (near) half-pixel ties and voxels on both
+    sides of the camera plane: 1 mm voxels seen head-on by fx = 500 put every
+    other column on pixel coordinate k + 0.5 up to float rounding, so the
+    reciprocal shortcut must hand them to the reference's division; a camera
+    placed inside the grid gives tiles that straddle pc_z = 0."""
+    dims = (256, 192, 64)
+    origin = np.array([-0.32, -0.24, 1.0], np.float32)
+    vs, trunc = np.float32(0.001), np.float32(0.005)
+    K = np.array([[500, 0, 320], [0, 500, 240], [0, 0, 1]], np.float32)
+    H, W = 480, 640
+    poses = np.tile(np.eye(4, dtype=np.float32), (3, 1, 1))
+    poses[1, 2, 3] = 1.03  # camera inside the grid: slices on both sides of it
+    a = np.float32(0.002)  # a slight turn about y
+    poses[2, :3, :3] = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]],
+                                np.float32)
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float32)
+    depths = np.stack([
+        np.full((H, W), 1.03, np.float32),
+        np.float32(0.02) + np.float32(1e-5) * xx,
+        np.float32(1.01) + np.float32(4e-5) * (xx + yy),
+    ])
+    depths[:, ::7, ::5] = 0.0  # dropouts
+    print(f"case rounding boundaries dims={dims} fused={fused}", flush=True)
+    got = _gpu_fuse(torch_gpu, K, poses, depths, dims, origin, vs, trunc, fused)
+    ref = _oracle_fuse(K, poses, depths, dims, origin, vs, trunc)
+    assert ref[1].sum() > 100000
+    _assert_same(got, ref)
+
+
 def test_full_size_grid_bit_exact(torch_gpu):
     """The benchmark's 256^3 grid at 1 cm (first 12 of its 50 frames)."""
     from tdm import synth
