@@ -166,17 +166,12 @@ void tdm_fusion::issue_loop() {
     }
 }
 
-extern "C" {
+namespace {
 
-int tdm_fusion_create(tdm_ctx *ctx, const float cam_K[9], int im_height, int im_width,
-                      const int dims[3], const float origin[3], float voxel_size,
-                      float trunc_margin, int threads, tdm_fusion **out) {
-    tdm::clear_error();
-    TDM_CHECK(cam_K && dims && origin && out, TDM_ERR_INVALID_ARG, "null pointer argument");
-    TDM_CHECK(im_height > 0 && im_width > 0 && dims[0] > 0 && dims[1] > 0 && dims[2] > 0,
-              TDM_ERR_INVALID_ARG, "bad sizes");
-    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
-    auto f = std::make_unique<tdm_fusion>();
+// Device and pinned resources of a new pipeline (no threads yet).
+int fusion_setup(tdm_ctx *ctx, tdm_fusion *f, const float cam_K[9], int im_height, int im_width,
+                 const int dims[3], const float origin[3], float voxel_size, float trunc_margin,
+                 int threads) {
     f->device = ctx->device;
     f->H = im_height;
     f->W = im_width;
@@ -204,7 +199,33 @@ int tdm_fusion_create(tdm_ctx *ctx, const float cam_K[9], int im_height, int im_
         TDM_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     }
     TDM_HIP(hipStreamSynchronize(f->stream));
-    tdm_fusion *raw = f.release();
+    return TDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdm_fusion_create(tdm_ctx *ctx, const float cam_K[9], int im_height, int im_width,
+                      const int dims[3], const float origin[3], float voxel_size,
+                      float trunc_margin, int threads, tdm_fusion **out) {
+    tdm::clear_error();
+    TDM_CHECK(cam_K && dims && origin && out, TDM_ERR_INVALID_ARG, "null pointer argument");
+    TDM_CHECK(im_height > 0 && im_width > 0 && dims[0] > 0 && dims[1] > 0 && dims[2] > 0,
+              TDM_ERR_INVALID_ARG, "bad sizes");
+    TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
+    tdm_fusion *f = new tdm_fusion;
+    // a failure part-way releases what was created (tdm_fusion_destroy frees
+    // the non-null members and has no threads to join yet)
+    const int st = fusion_setup(ctx, f, cam_K, im_height, im_width, dims, origin, voxel_size,
+                                trunc_margin, threads);
+    if (st != TDM_OK) {
+        const std::string msg = tdm_last_error();
+        tdm_fusion_destroy(f);
+        return tdm::fail(st, "%s", msg.c_str());
+    }
+    tdm_fusion *raw = f;
+    const int nthreads = (int)raw->slots.size() - 2;
     for (int t = 0; t < nthreads; ++t) raw->decoders.emplace_back([raw] { raw->decode_loop(); });
     raw->issuer = std::thread([raw] { raw->issue_loop(); });
     *out = raw;

@@ -295,3 +295,16 @@ def test_fusion_pipeline_equals_serial(torch_gpu, tmp_path):
     ref = _gpu_fuse(torch, K, poses, depths, dims, origin, vs, trunc, False)
     shape = (dims[2], dims[1], dims[0])
     _assert_same((t.cpu().numpy().reshape(shape), w.cpu().numpy().reshape(shape)), ref)
+
+
+def test_fusion_create_failure_is_clean(torch_gpu):
+    """A grid that cannot be allocated fails tdm_fusion_create with an error
+    (the partly built pipeline is released) and a normal pipeline still works."""
+    K = np.array([[585, 0, 320], [0, 585, 240], [0, 0, 1]], np.float32)
+    origin = np.zeros(3, np.float32)
+    print("case fusion create with an unallocatable grid", flush=True)
+    with pytest.raises(tdm.TdmError):
+        tdm.Fusion(K, 480, 640, (65536, 65536, 65536), origin, 0.01, 0.05, threads=1)
+    fus = tdm.Fusion(K, 480, 640, (32, 32, 32), origin, 0.01, 0.05, threads=1)
+    fus.wait()
+    fus.close()
