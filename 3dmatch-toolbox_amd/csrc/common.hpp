@@ -18,12 +18,16 @@ namespace tdm {
 int fail(int status, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
+// A failed HIP call also leaves its code in HIP's per-thread last error,
+// which the next launch check would read; it is cleared here.
 #define TDM_HIP(expr)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
+        if (e_ != hipSuccess) {                                                            \
+            (void)hipGetLastError();                                                       \
             return ::tdm::fail(TDM_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                \
                                hipGetErrorString(e_), __FILE__, __LINE__);                 \
+        }                                                                                  \
     } while (0)
 
 #define TDM_CHECK(cond, status, ...)                                                       \

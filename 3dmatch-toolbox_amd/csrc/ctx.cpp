@@ -25,6 +25,10 @@ static std::mutex g_ctx_mu;
 static std::map<int, tdm_ctx *> g_default_ctx;
 
 int resolve_ctx(tdm_ctx *in, tdm_ctx **out) {
+    // an error a caller's earlier HIP call left behind (and already reported
+    // to that caller) must not be taken for a failure of this entry point's
+    // launches
+    (void)hipGetLastError();
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
         (void)hipGetLastError();

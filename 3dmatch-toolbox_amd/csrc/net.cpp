@@ -436,6 +436,7 @@ int resolve_timing(tdm_net &net) {
 // The device is bound at the first forward (a net can be built and given
 // weights without a GPU); later calls make it current again.
 int bind_device(tdm_net *net) {
+    (void)hipGetLastError();  // a caller's stale error is not this call's failure
     if (net->device < 0) {
         tdm_ctx *c = net->ctx;
         TDM_TRY(tdm::resolve_ctx(c, &c));
