@@ -514,8 +514,11 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         s0 = (int)((long)nc * tz / p.splits) * CG;
         s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
     }
-    uint4 ra[A_PT], rb[B_PT];
-    auto fetch = [&](int s) {
+    // Two register sets: the loads of stage s+2 are issued before the MFMAs
+    // of stage s, and stage s+1 (loaded one stage earlier) goes to LDS after
+    // them, so a global load has two stages of compute to arrive.
+    uint4 ra0[A_PT], rb0[B_PT], ra1[A_PT], rb1[B_PT];
+    auto fetch = [&](int s, uint4 (&ra)[A_PT], uint4 (&rb)[B_PT]) {
         const int tap = s / cps;
         const int c0 = (s - tap * cps) * SK;
         int kz = 0, ky = 0, kx = 0;
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
 #pragma unroll
         for (int j = 0; j < B_PT; ++j) rb[j] = ld16(b_src[j] + boff);
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](int buf, const uint4 (&ra)[A_PT], const uint4 (&rb)[B_PT]) {
 #pragma unroll
         for (int j = 0; j < A_PT; ++j) *reinterpret_cast<uint4 *>(&lds[buf][a_dst[j]]) = ra[j];
 #pragma unroll
@@ -545,29 +548,38 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const int r16 = lane & 15, g = lane >> 4;
-    if (s0 < s1) {
-        fetch(s0);
-        stash(0);
+    auto compute = [&](int buf) {
+        const unsigned char *A = &lds[buf][(wm * WTM + r16) * kLdsRow + g * 16];
+        const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h) {
+            uint4 a[IB], b[JB];
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+                a[i] = *reinterpret_cast<const uint4 *>(A + 16 * i * kLdsRow + h * 64);
+#pragma unroll
+            for (int j = 0; j < JB; ++j)
+                b[j] = *reinterpret_cast<const uint4 *>(B + 16 * j * kLdsRow + h * 64);
+            Mfma<T>::chunk(acc, a, b);
+        }
+    };
+    // stage s from LDS; `held` = stage s+1 in registers, `next` receives s+2
+    auto step = [&](int s, const uint4 (&ha)[A_PT], const uint4 (&hb)[B_PT], uint4 (&na)[A_PT],
+                    uint4 (&nb)[B_PT]) {
+        const int buf = (s - s0) & 1;
+        if (s + 2 < s1) fetch(s + 2, na, nb);
+        compute(buf);
+        if (s + 1 < s1) stash(buf ^ 1, ha, hb);
         __syncthreads();
-        for (int s = s0; s < s1; ++s) {
-            const int buf = (s - s0) & 1;
-            const bool more = s + 1 < s1;
-            if (more) fetch(s + 1);
-            const unsigned char *A = &lds[buf][(wm * WTM + r16) * kLdsRow + g * 16];
-            const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
-#pragma unroll
-            for (int h = 0; h < HALVES; ++h) {
-                uint4 a[IB], b[JB];
-#pragma unroll
-                for (int i = 0; i < IB; ++i)
-                    a[i] = *reinterpret_cast<const uint4 *>(A + 16 * i * kLdsRow + h * 64);
-#pragma unroll
-                for (int j = 0; j < JB; ++j)
-                    b[j] = *reinterpret_cast<const uint4 *>(B + 16 * j * kLdsRow + h * 64);
-                Mfma<T>::chunk(acc, a, b);
-            }
-            if (more) stash(buf ^ 1);
-            __syncthreads();
+    };
+    if (s0 < s1) {
+        fetch(s0, ra0, rb0);
+        stash(0, ra0, rb0);
+        if (s0 + 1 < s1) fetch(s0 + 1, ra1, rb1);
+        __syncthreads();
+        for (int s = s0; s < s1; s += 2) {
+            step(s, ra1, rb1, ra0, rb0);
+            if (s + 1 < s1) step(s + 1, ra0, rb0, ra1, rb1);
         }
     }
     conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
