@@ -37,6 +37,23 @@ def test_invalid_arguments_are_reported_not_fatal():
     assert st == tdm.ERR_INVALID_ARG and b"null" in L.tdm_last_error()
 
 
+def test_integrate_size_limits_rejected_on_the_host():
+    """The TSDF kernels address a grid slice and the depth image with 32-bit
+    byte offsets; larger inputs are refused before any device work."""
+    L = tdm.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks come first
+    f = ctypes.c_float
+    st = L.tdm_integrate(None, dummy, dummy, dummy, 480, 640, 70000, 70000, 1, f(0), f(0), f(0),
+                         f(0.01), f(0.05), dummy, dummy, None)
+    assert st == tdm.ERR_INVALID_ARG and b"slice" in L.tdm_last_error()
+    st = L.tdm_integrate(None, dummy, dummy, dummy, 40000, 40000, 8, 8, 8, f(0), f(0), f(0),
+                         f(0.01), f(0.05), dummy, dummy, None)
+    assert st == tdm.ERR_INVALID_ARG and b"image" in L.tdm_last_error()
+    st = L.tdm_integrate(None, dummy, dummy, dummy, 480, 640, 8, 8, 8, f(0), f(0), f(0),
+                         f(0.0), f(0.05), dummy, dummy, None)
+    assert st == tdm.ERR_INVALID_ARG
+
+
 def _shapes(net):
     return [(n, net.layer_info(i)) for i, n in enumerate(net.layer_names)]
 
