@@ -172,16 +172,39 @@ __global__ void pack_bits_kernel(const float *__restrict__ occ, int dx, long row
 
 // ---- TDF tile kernel --------------------------------------------------------------
 
-// LDS of one tile: halo bit words (dead after pass x, so pass y's output
-// reuses them -- keeps two workgroups per CU at R = 5), pass-x output, table.
+// LDS of one tile: halo bit words, pass-y output, table.  Pass x lives in
+// registers (a wave runs passes x and y over whole z-slabs of the halo), so
+// a workgroup holds about 38 KB at R = 5 and four fit on a CU.
 template <int R>
 struct TileLds {
     static constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
-    static constexpr int kSwBytes = HZ * HY * 4 * 4, kS2Bytes = HZ * kTY * kTX;
-    unsigned char a[kSwBytes > kS2Bytes ? kSwBytes : kS2Bytes] __attribute__((aligned(16)));
-    unsigned char s1[HZ * HY * kTX];
+    unsigned sw[HZ * HY * 4];
+    unsigned char s2[HZ * kTY * kTX];
     float lut[kLutSize];
 };
+
+// Squared distance (or kNone) along x from voxel `lane` of the tile to the
+// nearest set bit of halo row words w0..w3 (bit 0 of w0 = tile x - 32).
+template <int R>
+__device__ inline unsigned char xpass(unsigned w0, unsigned w1, unsigned w2, unsigned w3, int lane) {
+    if (!(w0 | w1 | w2 | w3)) return kNone;
+    const unsigned long long lo = ((unsigned long long)w1 << 32) | w0;
+    const unsigned long long hi = ((unsigned long long)w3 << 32) | w2;
+    // Window of 2R+1 bits starting at local bit lane + 32 - R.
+    const int s = lane + 32 - R;
+    unsigned long long win;
+    if (s == 0) win = lo;
+    else if (s < 64) win = (lo >> s) | (hi << (64 - s));
+    else win = hi >> (s - 64);
+    win &= (1ull << (2 * R + 1)) - 1;
+    if (!win) return kNone;
+    const unsigned long long right = win >> R;              // bits at >= x
+    const unsigned long long left = win & ((2ull << R) - 1);  // bits at <= x
+    int d = 255;
+    if (right) d = __builtin_ctzll(right);
+    if (left) d = min(d, R - (63 - __builtin_clzll(left)));
+    return (unsigned char)(d * d);
+}
 
 // Truncated squared-distance transform of the 64x16x16 tile at (X0, Y0, Z0).
 template <int R>
@@ -189,9 +212,8 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
                                 int dz, const Lut &lut, float *__restrict__ tdf, int X0, int Y0,
                                 int Z0) {
     constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
-    unsigned *sw = reinterpret_cast<unsigned *>(L.a);
-    unsigned char *s2 = L.a;
-    unsigned char *s1 = L.s1;
+    unsigned *sw = L.sw;
+    unsigned char *s2 = L.s2;
     float *slut = L.lut;
 
     const int lane = threadIdx.x & 63;
@@ -214,47 +236,26 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
     }
     const bool empty = __syncthreads_or(seen != 0u) == 0;
 
-    // Pass x over every halo row: nearest set bit within R along x.
-    for (int row = wave; !empty && row < HY * HZ; row += kTdfThreads / 64) {
-        unsigned char f = kNone;
-        {
-            const unsigned w0 = sw[row * 4 + 0], w1 = sw[row * 4 + 1];
-            const unsigned w2 = sw[row * 4 + 2], w3 = sw[row * 4 + 3];
-            if (w0 | w1 | w2 | w3) {
-                unsigned long long lo = ((unsigned long long)w1 << 32) | w0;
-                unsigned long long hi = ((unsigned long long)w3 << 32) | w2;
-                // Window of 2R+1 bits starting at local bit lane + 32 - R.
-                const int s = lane + 32 - R;
-                unsigned long long win;
-                if (s == 0) win = lo;
-                else if (s < 64) win = (lo >> s) | (hi << (64 - s));
-                else win = hi >> (s - 64);
-                win &= (1ull << (2 * R + 1)) - 1;
-                if (win) {
-                    unsigned long long right = win >> R;  // bits at >= x
-                    unsigned long long left = win & ((2ull << R) - 1);  // bits at <= x
-                    int d = 255;
-                    if (right) d = __builtin_ctzll(right);
-                    if (left) d = min(d, R - (63 - __builtin_clzll(left)));
-                    f = (unsigned char)(d * d);
-                }
-            }
-        }
-        s1[row * kTX + lane] = f;
-    }
-    __syncthreads();
-
     if (!empty) {
-        // Pass y: rows (zz in halo, y in tile).
-        for (int row = wave; row < HZ * kTY; row += kTdfThreads / 64) {
-            const int y = row % kTY, zz = row / kTY;
-            int best = kNone;
+        // Passes x and y over the halo z-slab zz: pass x of its HY rows in
+        // registers, then min over |d| <= R of f(y + d) + d^2 for the tile's y.
+        for (int zz = wave; zz < HZ; zz += kTdfThreads / 64) {
+            unsigned char f[HY];
 #pragma unroll
-            for (int d = -R; d <= R; ++d) {
-                int v = s1[(zz * HY + y + R + d) * kTX + lane];
-                if (v != kNone) best = min(best, v + d * d);
+            for (int yy = 0; yy < HY; ++yy) {
+                const unsigned *w = &sw[(zz * HY + yy) * 4];
+                f[yy] = xpass<R>(w[0], w[1], w[2], w[3], lane);
             }
-            s2[row * kTX + lane] = (unsigned char)min(best, (int)kNone);
+#pragma unroll
+            for (int y = 0; y < kTY; ++y) {
+                int best = kNone;
+#pragma unroll
+                for (int d = -R; d <= R; ++d) {
+                    const int v = f[y + R + d];
+                    if (v != kNone) best = min(best, v + d * d);
+                }
+                s2[(zz * kTY + y) * kTX + lane] = (unsigned char)min(best, (int)kNone);
+            }
         }
         __syncthreads();
     }

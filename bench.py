@@ -204,6 +204,44 @@ def tsdf_bench(torch, reps=3):
     return out
 
 
+def tdf_bench(torch, pts, reps=5):
+    """TDF of the descriptor workload's fragment (occupancy scatter + the
+    truncated distance transform, one call of tdm_tdf_from_points) in both
+    decode modes, and the accurate point-distance TDF.  Algorithmic bytes:
+    12 per point read + 4 per TDF voxel written."""
+    import numpy as np
+
+    import tdm
+
+    vs, tr = np.float32(0.01), np.float32(0.05)
+    dims, origin = tdm.grid_params(pts, vs, 15)
+    nvox = dims[0] * dims[1] * dims[2]
+    tdf = torch.empty(nvox, dtype=torch.float32, device=pts.device)
+    algo = 12.0 * pts.shape[0] + 4.0 * nvox
+
+    def timed(fn):
+        times = []
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        return float(np.median(times[1:]))
+
+    out = {"dims": dims, "points": int(pts.shape[0]), "voxels": nvox, "algorithmic_bytes": algo}
+    for name, dec in (("exact_decode", tdm.DECODE_EXACT), ("reference_decode", tdm.DECODE_REFERENCE)):
+        ms = timed(lambda: tdm.tdf_from_points(pts, dims, origin, vs, tr, tdf, decode=dec))
+        out[name] = {"ms": ms, "gvoxels_per_s": nvox / (ms / 1e3) / 1e9,
+                     "hbm_gbs": algo / (ms / 1e3) / 1e9,
+                     "hbm_frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    ms = timed(lambda: tdm.tdf_accurate(pts, dims, origin, vs, tr, tdf))
+    out["accurate"] = {"ms": ms, "gvoxels_per_s": nvox / (ms / 1e3) / 1e9,
+                       "hbm_frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def tsdf_scenes_bench(torch, n_scenes=16, reps=3):
     """Batched TSDF: n_scenes synthetic scenes (sphere seeds 50..), 50 frames
     each, fused into n_scenes 256^3 grids in one launch."""
@@ -763,6 +801,9 @@ def main() -> int:
         log("precision study")
         result["precision"] = precision_study(torch, js, wt, weights)
         torch.cuda.empty_cache()
+        log("tdf")
+        result["tdf"] = tdf_bench(torch, torch.from_numpy(cloud).to("cuda:0"))
+        result["tdf"]["config"] = "the 200k-point fragment of the descriptor workload, 0.01 m voxels, 5-voxel truncation, 15-voxel padding"
         log("tsdf fusion")
         result["tsdf"] = tsdf_bench(torch)
         result["tsdf"]["config"] = "50 synthetic 640x480 depth frames into a 256^3 grid, 0.01 m voxels, truncation 0.05 m"
