@@ -207,10 +207,13 @@ __device__ inline unsigned char xpass(unsigned w0, unsigned w1, unsigned w2, uns
 }
 
 // Truncated squared-distance transform of the 64x16x16 tile at (X0, Y0, Z0).
+// With `flags` (one byte per tile: any occupied voxel inside it), a tile
+// whose 27 neighbouring tiles -- which contain its halo, R < 16 -- hold no
+// occupied voxel writes zeros without loading the halo.
 template <int R>
 __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits, int dx, int dy,
                                 int dz, const Lut &lut, float *__restrict__ tdf, int X0, int Y0,
-                                int Z0) {
+                                int Z0, const unsigned char *__restrict__ flags) {
     constexpr int HY = kTY + 2 * R, HZ = kTZ + 2 * R;
     unsigned *sw = L.sw;
     unsigned char *s2 = L.s2;
@@ -220,6 +223,26 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
     const int wave = threadIdx.x >> 6;
     const long rw = row_words(dx);
     const long wbase = X0 >> 5;  // first word of the tile in a row
+
+    if (flags) {
+        bool near = false;
+        if (threadIdx.x < 27) {
+            const int ntx = (dx + kTX - 1) / kTX, nty = (dy + kTY - 1) / kTY, ntz = (dz + kTZ - 1) / kTZ;
+            const int tx = X0 / kTX + (int)threadIdx.x % 3 - 1;
+            const int ty = Y0 / kTY + ((int)threadIdx.x / 3) % 3 - 1;
+            const int tz = Z0 / kTZ + (int)threadIdx.x / 9 - 1;
+            if (tx >= 0 && tx < ntx && ty >= 0 && ty < nty && tz >= 0 && tz < ntz)
+                near = flags[((long)tz * nty + ty) * ntx + tx] != 0;
+        }
+        if (__syncthreads_or(near) == 0) {
+            const int x = X0 + lane;
+            for (int row = wave; row < kTY * kTZ; row += kTdfThreads / 64) {
+                const int gy = Y0 + row % kTY, gz = Z0 + row / kTY;
+                if (gy < dy && gz < dz && x < dx) tdf[((long)gz * dy + gy) * dx + x] = 0.0f;
+            }
+            return;
+        }
+    }
 
     for (int i = threadIdx.x; i < kLutSize; i += kTdfThreads) slut[i] = lut.v[i];
     // All halo words at once (independent loads, so their latencies overlap).
@@ -246,14 +269,13 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
                 const unsigned *w = &sw[(zz * HY + yy) * 4];
                 f[yy] = xpass<R>(w[0], w[1], w[2], w[3], lane);
             }
+            // kNone (255) exceeds every reachable sum (2 R^2 <= 128), so it needs
+            // no test: a result >= kNone means nothing within reach.
 #pragma unroll
             for (int y = 0; y < kTY; ++y) {
                 int best = kNone;
 #pragma unroll
-                for (int d = -R; d <= R; ++d) {
-                    const int v = f[y + R + d];
-                    if (v != kNone) best = min(best, v + d * d);
-                }
+                for (int d = -R; d <= R; ++d) best = min(best, (int)f[y + R + d] + d * d);
                 s2[(zz * kTY + y) * kTX + lane] = (unsigned char)min(best, (int)kNone);
             }
         }
@@ -270,10 +292,8 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
         if (!empty) {
             int best = kNone;
 #pragma unroll
-            for (int d = -R; d <= R; ++d) {
-                int v = s2[((zl + R + d) * kTY + y) * kTX + lane];
-                if (v != kNone) best = min(best, v + d * d);
-            }
+            for (int d = -R; d <= R; ++d)
+                best = min(best, (int)s2[((zl + R + d) * kTY + y) * kTX + lane] + d * d);
             if (best < kLutSize) val = slut[best];
         }
         tdf[((long)gz * dy + gy) * dx + x] = val;
@@ -283,10 +303,32 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
 template <int R>
 __global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *__restrict__ bits,
                                                                int dx, int dy, int dz, Lut lut,
-                                                               float *__restrict__ tdf) {
+                                                               float *__restrict__ tdf,
+                                                               const unsigned char *__restrict__ flags) {
     __shared__ TileLds<R> L;
     tdf_tile<R>(L, bits, dx, dy, dz, lut, tdf, blockIdx.x * kTX, blockIdx.y * kTY,
-                blockIdx.z * kTZ);
+                blockIdx.z * kTZ, flags);
+}
+
+// One byte per 64x16x16 tile: whether any voxel of the tile is occupied
+// (one wave per tile ORs the tile's 16 x 16 rows of two words).
+__global__ void tile_flags_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
+                                  unsigned char *__restrict__ flags) {
+    const int ntx = (dx + kTX - 1) / kTX, nty = (dy + kTY - 1) / kTY, ntz = (dz + kTZ - 1) / kTZ;
+    const long tile = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (tile >= (long)ntx * nty * ntz) return;  // whole wave
+    const int tx = (int)(tile % ntx), ty = (int)((tile / ntx) % nty), tz = (int)(tile / ((long)ntx * nty));
+    const long rw = row_words(dx);
+    const long w0 = (long)tx * (kTX / 32);
+    unsigned v = 0u;
+    for (int i = lane; i < kTY * kTZ * 2; i += 64) {
+        const int row = i >> 1, wi = i & 1;
+        const int y = ty * kTY + row % kTY, z = tz * kTZ + row / kTY;
+        if (y < dy && z < dz && w0 + wi < rw) v |= bits[((long)z * dy + y) * rw + w0 + wi];
+    }
+    const bool any = __ballot(v != 0u) != 0ull;
+    if (lane == 0) flags[tile] = any ? 1 : 0;
 }
 
 // ---- many clouds in one launch per step ---------------------------------------
@@ -346,7 +388,7 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_batch_kernel(
     const int ty = (int)((lt / cd.tiles[0]) % cd.tiles[1]);
     const int tz = (int)(lt / ((long)cd.tiles[0] * cd.tiles[1]));
     tdf_tile<R>(L, bits + cd.word_off, cd.dims[0], cd.dims[1], cd.dims[2], lut, cd.tdf, tx * kTX,
-                ty * kTY, tz * kTZ);
+                ty * kTY, tz * kTZ, nullptr);
 }
 
 // float32 decode of core/demo.cu:32-34.
@@ -531,24 +573,32 @@ int search_radius(float vs, float trunc, int *r) {
 
 template <int R>
 void launch_tile(const unsigned *bits, int dx, int dy, int dz, const Lut &lut, float *tdf,
-                 hipStream_t s) {
+                 const unsigned char *flags, hipStream_t s) {
     dim3 grid((dx + kTX - 1) / kTX, (dy + kTY - 1) / kTY, (dz + kTZ - 1) / kTZ);
-    tdf_tile_kernel<R><<<grid, kTdfThreads, 0, s>>>(bits, dx, dy, dz, lut, tdf);
+    tdf_tile_kernel<R><<<grid, kTdfThreads, 0, s>>>(bits, dx, dy, dz, lut, tdf, flags);
 }
 
-// TDF from a bit grid already in scratch.
+long num_tiles(int dx, int dy, int dz) {
+    return (long)((dx + kTX - 1) / kTX) * ((dy + kTY - 1) / kTY) * ((dz + kTZ - 1) / kTZ);
+}
+
+// TDF from a bit grid already in scratch; `flags` (num_tiles bytes of
+// scratch) receives the per-tile occupancy.
 int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decode, float *tdf,
-                  hipStream_t s) {
+                  unsigned char *flags, hipStream_t s) {
     Lut lut = make_lut(r);
+    const long nt = num_tiles(dx, dy, dz);
+    tile_flags_kernel<<<(unsigned)((nt + 3) / 4), 256, 0, s>>>(bits, dx, dy, dz, flags);
+    TDM_LAUNCHED();
     switch (r) {
-    case 1: launch_tile<1>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 2: launch_tile<2>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 3: launch_tile<3>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 4: launch_tile<4>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 5: launch_tile<5>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 6: launch_tile<6>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 7: launch_tile<7>(bits, dx, dy, dz, lut, tdf, s); break;
-    case 8: launch_tile<8>(bits, dx, dy, dz, lut, tdf, s); break;
+    case 1: launch_tile<1>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 2: launch_tile<2>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 3: launch_tile<3>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 4: launch_tile<4>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 5: launch_tile<5>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 6: launch_tile<6>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 7: launch_tile<7>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 8: launch_tile<8>(bits, dx, dy, dz, lut, tdf, flags, s); break;
     default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", r);
     }
     TDM_LAUNCHED();
@@ -561,12 +611,14 @@ int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decod
     return TDM_OK;
 }
 
+// Bit grid (cleared) and, after it, the per-tile flags, in scratch slot 0.
 int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, unsigned **bits, size_t *words,
-                 hipStream_t s) {
+                 unsigned char **flags, hipStream_t s) {
     *words = (size_t)row_words(dx) * dy * dz;
     void *p = nullptr;
-    TDM_TRY(tdm::ctx_scratch(ctx, 0, *words * sizeof(unsigned), &p, s));
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, *words * sizeof(unsigned) + num_tiles(dx, dy, dz), &p, s));
     *bits = static_cast<unsigned *>(p);
+    *flags = reinterpret_cast<unsigned char *>(*bits + *words);
     unsigned blocks = (unsigned)std::min<size_t>(tdm::blocks_for(*words, 256), 16384);
     fill_u32_kernel<<<blocks, 256, 0, s>>>(*bits, *words, 0u);
     TDM_LAUNCHED();
@@ -653,13 +705,14 @@ int tdm_compute_tdf(tdm_ctx *ctx, const float *occ, float *tdf, int dim_x, int d
     hipStream_t s = tdm::as_stream(stream);
     unsigned *bits = nullptr;
     size_t words = 0;
-    TDM_TRY(bits_scratch(ctx, dim_x, dim_y, dim_z, &bits, &words, s));
+    unsigned char *flags = nullptr;
+    TDM_TRY(bits_scratch(ctx, dim_x, dim_y, dim_z, &bits, &words, &flags, s));
     const long rows = (long)dim_y * dim_z;
     const long pairs = (row_words(dim_x) + 1) / 2;
     const long waves = rows * pairs;
     pack_bits_kernel<<<tdm::blocks_for(waves * 64, 256), 256, 0, s>>>(occ, dim_x, rows, bits);
     TDM_LAUNCHED();
-    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, s);
+    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, flags, s);
 }
 
 int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
@@ -678,14 +731,15 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     hipStream_t s = tdm::as_stream(stream);
     unsigned *bits = nullptr;
     size_t words = 0;
-    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], &bits, &words, s));
+    unsigned char *flags = nullptr;
+    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], &bits, &words, &flags, s));
     if (num_pts > 0) {
         scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
             bits);
         TDM_LAUNCHED();
     }
-    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, s);
+    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, flags, s);
 }
 
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
