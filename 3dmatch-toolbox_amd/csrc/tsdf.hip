@@ -176,7 +176,7 @@ __device__ inline int pixel_of(const Cam &c, const FrameBounds &f, float pcx, fl
     const float rx = rintf(vx), ry = rintf(vy);
     const bool cert = pcz >= 0x1p-64f && fabsf(vx - rx) < f.hx && fabsf(vy - ry) < f.hy;
     const bool in = rx >= 0.0f && rx < f.Wf && ry >= 0.0f && ry < f.Hf;
-    int pix = (cert && in) ? (int)fmaf(ry, f.Wf, rx) : -1;  // exact: integers below 2^24
+    int pix = (cert && in) ? (int)ry * W + (int)rx : -1;
     if (!cert && pcz > 0.0f) {
         const int ix = exact_index(pcx, pcz, c.fx, c.cx, W);
         const int iy = exact_index(pcy, pcz, c.fy, c.cy, H);

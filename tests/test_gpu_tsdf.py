@@ -308,3 +308,25 @@ def test_fusion_create_failure_is_clean(torch_gpu):
     fus = tdm.Fusion(K, 480, 640, (32, 32, 32), origin, 0.01, 0.05, threads=1)
     fus.wait()
     fus.close()
+
+
+def test_large_image_pixel_indices(torch_gpu):
+    """A 4100 x 4100 frame (more pixels than float32 counts exactly, 2^24):
+    voxels that project past pixel 2^24 must read their own pixel.  Depth and
+    dropouts change from pixel to pixel inside the truncation band, so a
+    neighbouring pixel would show in the grid."""
+    H = W = 4100
+    K = np.array([[3000, 0, 2050], [0, 3000, 4050], [0, 0, 1]], np.float32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    depth = (np.float32(0.955) + np.float32(0.002) * ((xx * 7 + yy * 13) % 7)).astype(np.float32)
+    depth[(xx * 3 + yy) % 4 == 0] = 0.0
+    depth = depth[None]
+    poses = np.eye(4, dtype=np.float32)[None]
+    dims = (256, 32, 40)
+    origin = np.array([-0.064, 0.0, 0.95], np.float32)  # rows up to pixel row 4099
+    vs, trunc = np.float32(0.0005), np.float32(0.01)
+    print("case 4100x4100 frame", flush=True)
+    got = _gpu_fuse(torch_gpu, K, poses, depth, dims, origin, vs, trunc, False)
+    ref = _oracle_fuse(K, poses, depth, dims, origin, vs, trunc)
+    assert ref[1].sum() > 10000
+    _assert_same(got, ref)
