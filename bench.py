@@ -683,6 +683,8 @@ def main() -> int:
                     help="precision of the headline pipeline (default fp32, the plain path)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving sweep (e.g. under a profiler)")
+    ap.add_argument("--only", nargs="+", choices=["tsdf", "tdf"],
+                    help="run only these workloads and print their figures (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (reference for HBM counters)")
     args = ap.parse_args()
@@ -711,6 +713,17 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     tdm.lib()
+    if args.only:
+        out = {}
+        if "tdf" in args.only:
+            log("tdf")
+            cloud, _, _ = descriptor_workload(200_000, 5000)
+            out["tdf"] = tdf_bench(torch, torch.from_numpy(cloud).to("cuda:0"))
+        if "tsdf" in args.only:
+            log("tsdf fusion")
+            out["tsdf"] = tsdf_bench(torch)
+        print(json.dumps(out), flush=True)
+        return 0
     log("building inputs")
     n_kp = 5000
     cloud, keypts, weights = descriptor_workload(200_000, n_kp)
