@@ -102,11 +102,12 @@ __device__ inline float4 slice_term(const Cam &c, float oz, float vs, int z) {
 // exact one, so a tile with all corners beyond a margin that covers the
 // float evaluation of the form is rejected by the reference voxel by voxel.
 struct FrameBounds {
-    float hx, hy;          // certified when |va - rint(va)| < h
+    float h;               // certified when |va - rint(va)| < h on both axes
     float eta, zeta;
-    float mL, mR, mT, mB;  // margins of the four edge forms
+    float cL, cR, cT, cB;  // constant terms of the four edge forms
+    float mL, mR, mT, mB;  // their margins
     float Wf, Hf;
-    bool edges;            // fx > 0 and fy > 0 (the edge forms assume it)
+    int edges;             // fx > 0 and fy > 0 (the edge forms assume it)
 };
 
 __device__ inline FrameBounds frame_bounds(const Cam &c, int H, int W, int dx, int dy, int dz,
@@ -115,8 +116,9 @@ __device__ inline FrameBounds frame_bounds(const Cam &c, int H, int W, int dx, i
     const float Wf = (float)W, Hf = (float)H;
     const float acx = fabsf(c.cx), acy = fabsf(c.cy);
     // each bound is rounded up by a factor covering its own float evaluation
-    f.hx = 0.5f - (kU * (7.0f * (Wf + 1.0f + acx) + 2.0f * (Wf + 1.0f)) + 0x1p-20f) * 1.01f;
-    f.hy = 0.5f - (kU * (7.0f * (Hf + 1.0f + acy) + 2.0f * (Hf + 1.0f)) + 0x1p-20f) * 1.01f;
+    const float hx = 0.5f - (kU * (7.0f * (Wf + 1.0f + acx) + 2.0f * (Wf + 1.0f)) + 0x1p-20f) * 1.01f;
+    const float hy = 0.5f - (kU * (7.0f * (Hf + 1.0f + acy) + 2.0f * (Hf + 1.0f)) + 0x1p-20f) * 1.01f;
+    f.h = fminf(hx, hy);
     const float Sx = fabsf(ox) + (float)dx * vs + fabsf(c.tx);
     const float Sy = fabsf(oy) + (float)dy * vs + fabsf(c.ty);
     const float Sz = fabsf(oz) + (float)dz * vs + fabsf(c.tz);
@@ -128,16 +130,52 @@ __device__ inline FrameBounds frame_bounds(const Cam &c, int H, int W, int dx, i
     f.eta = eta * 1.01f;
     f.zeta = 1.02f * fmaxf(4.0f * eta, 8.0f * eta * (fmaxf(c.fx, c.fy) + fmaxf(Wf, Hf) +
                                                     fmaxf(acx, acy) + 1.0f));
-    const float cL = c.cx + 1.5f, cR = Wf + 0.5f - c.cx, cT = c.cy + 1.5f, cB = Hf + 0.5f - c.cy;
+    f.cL = c.cx + 1.5f;
+    f.cR = Wf + 0.5f - c.cx;
+    f.cT = c.cy + 1.5f;
+    f.cB = Hf + 0.5f - c.cy;
     const float g = 1.03f * (eta + 4.0f * kU * pmax);
-    f.mL = (c.fx + fabsf(cL)) * g + 1e-6f;
-    f.mR = (c.fx + fabsf(cR)) * g + 1e-6f;
-    f.mT = (c.fy + fabsf(cT)) * g + 1e-6f;
-    f.mB = (c.fy + fabsf(cB)) * g + 1e-6f;
+    f.mL = (c.fx + fabsf(f.cL)) * g + 1e-6f;
+    f.mR = (c.fx + fabsf(f.cR)) * g + 1e-6f;
+    f.mT = (c.fy + fabsf(f.cT)) * g + 1e-6f;
+    f.mB = (c.fy + fabsf(f.cB)) * g + 1e-6f;
     f.Wf = Wf;
     f.Hf = Hf;
     f.edges = c.fx > 0.0f && c.fy > 0.0f;
     return f;
+}
+
+// Everything the tile kernels need of one frame.  The multi-frame kernel gets
+// them from frame_consts_kernel, once per frame: this GPU has no scalar float
+// unit, so uniform float arithmetic costs a full vector instruction in every
+// wave that does it, and the kernel reads the block with scalar loads instead.
+// The one-frame kernel computes them itself (a separate launch per frame
+// costs more than the instructions it saves).
+struct alignas(16) FrameConst {
+    Cam c;
+    FrameBounds f;
+    float pad[2];
+};
+static_assert(sizeof(FrameConst) == 128, "FrameConst is two 64-byte scalar loads");
+
+// Entry i = scene * F + frame; origins (3 per scene) or the single origin.
+__global__ void frame_consts_kernel(const float *__restrict__ K, const float *__restrict__ cam2base,
+                                    int n, int F, int H, int W, int dx, int dy, int dz,
+                                    const float *__restrict__ origins, float ox, float oy,
+                                    float oz, float vs, FrameConst *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (origins) {
+        const int sc = i / F;
+        ox = origins[3 * sc + 0];
+        oy = origins[3 * sc + 1];
+        oz = origins[3 * sc + 2];
+    }
+    FrameConst r;
+    r.c = load_cam(K, cam2base + (size_t)i * 16);
+    r.f = frame_bounds(r.c, H, W, dx, dy, dz, ox, oy, oz, vs);
+    r.pad[0] = r.pad[1] = 0.0f;
+    out[i] = r;
 }
 
 // True when no voxel of the tile (x in [bx0, bx1] as world coordinates, row
@@ -152,9 +190,8 @@ __device__ inline bool tile_hidden(const Cam &c, const FrameBounds &f, float bx0
     const float px = q.sx + t.x, py = q.sy + t.y, pz = q.sz + t.z;
     if (__all(pz <= -2.5f * f.eta)) return true;  // behind the camera
     if (!f.edges || !__all(pz >= f.zeta + 1.5f * f.eta)) return false;
-    const float cL = c.cx + 1.5f, cR = f.Wf + 0.5f - c.cx, cT = c.cy + 1.5f, cB = f.Hf + 0.5f - c.cy;
-    const float hL = c.fx * px + cL * pz, hR = cR * pz - c.fx * px;
-    const float hT = c.fy * py + cT * pz, hB = cB * pz - c.fy * py;
+    const float hL = c.fx * px + f.cL * pz, hR = f.cR * pz - c.fx * px;
+    const float hT = c.fy * py + f.cT * pz, hB = f.cB * pz - c.fy * py;
     return __all(hL < -f.mL) || __all(hR < -f.mR) || __all(hT < -f.mT) || __all(hB < -f.mB);
 }
 
@@ -168,15 +205,19 @@ __device__ inline int exact_index(float pc, float pz, float f, float c, int n) {
 
 // Depth-image index of a voxel with camera coordinates (pcx, pcy, pcz), or
 // -1 when the reference skips it at depth-fusion/demo.cu:42-49.
-__device__ inline int pixel_of(const Cam &c, const FrameBounds &f, float pcx, float pcy,
-                               float pcz, int H, int W) {
+// cxv, cyv: c.cx, c.cy held in vector registers (an FMA reads one scalar).
+__device__ inline int pixel_of(const Cam &c, const FrameBounds &f, float cxv, float cyv,
+                               float pcx, float pcy, float pcz, int H, int W) {
     const float r = __builtin_amdgcn_rcpf(pcz);
-    const float vx = fmaf(c.fx, pcx * r, c.cx);
-    const float vy = fmaf(c.fy, pcy * r, c.cy);
+    const float vx = fmaf(c.fx, pcx * r, cxv);
+    const float vy = fmaf(c.fy, pcy * r, cyv);
     const float rx = rintf(vx), ry = rintf(vy);
-    const bool cert = pcz >= 0x1p-64f && fabsf(vx - rx) < f.hx && fabsf(vy - ry) < f.hy;
-    const bool in = rx >= 0.0f && rx < f.Wf && ry >= 0.0f && ry < f.Hf;
-    int pix = (cert && in) ? (int)ry * W + (int)rx : -1;
+    // one test for both axes against the smaller of the two bounds; bitwise
+    // ands keep the common path free of branches
+    const bool cert = (pcz >= 0x1p-64f) & (fmaxf(fabsf(vx - rx), fabsf(vy - ry)) < f.h);
+    const bool in = (rx >= 0.0f) & (rx < f.Wf) & (ry >= 0.0f) & (ry < f.Hf);
+    // in range: 0 <= ry < H < 2^24 and 0 <= rx < W < 2^24, ry*W + rx < 2^31
+    int pix = (cert & in) ? (int)__umul24((unsigned)ry, (unsigned)W) + (int)rx : -1;
     if (!cert && pcz > 0.0f) {
         const int ix = exact_index(pcx, pcz, c.fx, c.cx, W);
         const int iy = exact_index(pcy, pcz, c.fy, c.cy, H);
@@ -247,6 +288,8 @@ __global__ __launch_bounds__(64 * kRows) void integrate_frame_kernel(
     last.x = __shfl(mine.x, nz - 1); last.y = __shfl(mine.y, nz - 1); last.z = __shfl(mine.z, nz - 1);
     if (tile_hidden(c, f, ox + x0 * vs, ox + min(x0 + 63, dx - 1) * vs, by, first, last)) return;
     if (x >= dx) return;
+    float cxv = c.cx, cyv = c.cy;
+    asm volatile("" : "+v"(cxv), "+v"(cyv));
     const ColPart p = col_part(c, ox + x * vs, by);
     const unsigned plane = (unsigned)dx * (unsigned)dy;
     const unsigned boff = ((unsigned)y * (unsigned)dx + (unsigned)x) * 4u;
@@ -256,7 +299,7 @@ __global__ __launch_bounds__(64 * kRows) void integrate_frame_kernel(
     for (int j = 0; j < kZRun; ++j) {
         const float4 t = zt[j];
         pz[j] = p.sz + t.z;
-        const int v = pixel_of(c, f, p.sx + t.x, p.sy + t.y, pz[j], H, W);
+        const int v = pixel_of(c, f, cxv, cyv, p.sx + t.x, p.sy + t.y, pz[j], H, W);
         pix[j] = j < nz ? v : -1;
     }
     float d[kZRun];
@@ -296,18 +339,18 @@ __global__ __launch_bounds__(64 * kRows) void integrate_frame_kernel(
 // reference's, so the result is bit-identical.  Scene blockIdx.x / ntx of a
 // batch (ntx = x tiles per row) uses its own poses, frames, grid and origin.
 __global__ __launch_bounds__(64 * kRows) void integrate_frames_kernel(
-    const float *__restrict__ K, const float *__restrict__ cam2base,
-    const float *__restrict__ depths, int F, int H, int W, int dx, int dy, int dz, float ox,
-    float oy, float oz, float vs, float trunc, float *__restrict__ tsdf,
-    float *__restrict__ weight, const float *__restrict__ origins, int ntx) {
+    const FrameConst *__restrict__ fc, const float *__restrict__ depths, int F, int H, int W,
+    int dx, int dy, int dz, float ox, float oy, float oz, float vs, float trunc,
+    float *__restrict__ tsdf, float *__restrict__ weight, const float *__restrict__ origins,
+    int ntx) {
     __shared__ float4 slices[kRows][kZRun];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sc = blockIdx.x / ntx;
     const int y = blockIdx.y * kRows + wave;
     if (y >= dy) return;  // whole wave
     const unsigned plane = (unsigned)dx * (unsigned)dy;
+    fc += (size_t)sc * F;
     if (origins) {
-        cam2base += (size_t)sc * F * 16;
         depths += (size_t)sc * F * H * W;
         tsdf += (size_t)sc * plane * dz;
         weight += (size_t)sc * plane * dz;
@@ -328,15 +371,17 @@ __global__ __launch_bounds__(64 * kRows) void integrate_frames_kernel(
     float t[kZRun], w[kZRun];
     unsigned loaded = 0;
     for (int fr = 0; fr < F; ++fr) {
-        const Cam c = load_cam(K, cam2base + fr * 16);
+        const Cam c = fc[fr].c;
         const float4 mine = slice_term(c, oz, vs, z0 + min(lane, nz - 1));
         // the previous frame's reads of the table are done (same wave, in order)
         if (lane < kZRun) zt[lane] = mine;
-        const FrameBounds f = frame_bounds(c, H, W, dx, dy, dz, ox, oy, oz, vs);
+        const FrameBounds f = fc[fr].f;
         float4 first, last;
         first.x = __shfl(mine.x, 0); first.y = __shfl(mine.y, 0); first.z = __shfl(mine.z, 0);
         last.x = __shfl(mine.x, nz - 1); last.y = __shfl(mine.y, nz - 1); last.z = __shfl(mine.z, nz - 1);
         if (tile_hidden(c, f, bx0, bx1, by, first, last) || !live) continue;
+        float cxv = c.cx, cyv = c.cy;
+        asm volatile("" : "+v"(cxv), "+v"(cyv));
         const ColPart p = col_part(c, bx, by);
         const float *__restrict__ depth = depths + fr * frame_px;
         int pix[kZRun];
@@ -345,7 +390,7 @@ __global__ __launch_bounds__(64 * kRows) void integrate_frames_kernel(
         for (int j = 0; j < kZRun; ++j) {
             const float4 q = zt[j];
             pz[j] = p.sz + q.z;
-            const int v = pixel_of(c, f, p.sx + q.x, p.sy + q.y, pz[j], H, W);
+            const int v = pixel_of(c, f, cxv, cyv, p.sx + q.x, p.sy + q.y, pz[j], H, W);
             pix[j] = j < nz ? v : -1;
         }
         float d[kZRun];
@@ -424,6 +469,21 @@ int tile_grid(int dx, int dy, int dz, int scenes, dim3 *grid) {
     const long gy = (dy + kRows - 1) / kRows, gz = (dz + kZRun - 1) / kZRun;
     TDM_CHECK(gx < (1L << 31) && gy < 65536 && gz < 65536, TDM_ERR_INVALID_ARG, "grid too large");
     *grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)gz);
+    return TDM_OK;
+}
+
+// Per-frame constants of n = scenes * F frames into the context's scratch
+// slot 1, on stream s ahead of the tile kernel that reads them.
+int frame_consts(tdm_ctx *ctx, const float *K, const float *cam2base, int n, int F, int H, int W,
+                 int dx, int dy, int dz, const float *origins, float ox, float oy, float oz,
+                 float vs, hipStream_t s, const FrameConst **out) {
+    TDM_CHECK(n > 0 && n < (1 << 24), TDM_ERR_INVALID_ARG, "bad frame count %d", n);
+    void *p = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 1, sizeof(FrameConst) * (size_t)n, &p, s));
+    frame_consts_kernel<<<(n + 63) / 64, 64, 0, s>>>(K, cam2base, n, F, H, W, dx, dy, dz, origins,
+                                                     ox, oy, oz, vs, static_cast<FrameConst *>(p));
+    TDM_LAUNCHED();
+    *out = static_cast<const FrameConst *>(p);
     return TDM_OK;
 }
 
@@ -566,10 +626,13 @@ int tdm_integrate_frames(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     if (num_frames == 0) return TDM_OK;
     dim3 grid;
     TDM_TRY(tile_grid(dim_x, dim_y, dim_z, 1, &grid));
-    integrate_frames_kernel<<<grid, 64 * kRows, 0, tdm::as_stream(stream)>>>(
-        cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z,
-        origin_x, origin_y, origin_z, voxel_size, trunc_margin, tsdf, weight, nullptr,
-        (int)grid.x);
+    const hipStream_t s = tdm::as_stream(stream);
+    const FrameConst *fc = nullptr;
+    TDM_TRY(frame_consts(ctx, cam_K, cam2base, num_frames, num_frames, im_height, im_width, dim_x,
+                         dim_y, dim_z, nullptr, origin_x, origin_y, origin_z, voxel_size, s, &fc));
+    integrate_frames_kernel<<<grid, 64 * kRows, 0, s>>>(
+        fc, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z, origin_x, origin_y,
+        origin_z, voxel_size, trunc_margin, tsdf, weight, nullptr, (int)grid.x);
     TDM_LAUNCHED();
     return TDM_OK;
 }
@@ -589,9 +652,16 @@ int tdm_integrate_scenes(tdm_ctx *ctx, const float *cam_K, const float *cam2base
     if (num_frames == 0 || num_scenes == 0) return TDM_OK;
     dim3 grid;
     TDM_TRY(tile_grid(dim_x, dim_y, dim_z, num_scenes, &grid));
-    integrate_frames_kernel<<<grid, 64 * kRows, 0, tdm::as_stream(stream)>>>(
-        cam_K, cam2base, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z, 0.0f,
-        0.0f, 0.0f, voxel_size, trunc_margin, tsdf, weight, origins, (int)(grid.x / num_scenes));
+    TDM_CHECK((long)num_scenes * num_frames < (1L << 24), TDM_ERR_INVALID_ARG,
+              "too many scene frames");
+    const hipStream_t s = tdm::as_stream(stream);
+    const FrameConst *fc = nullptr;
+    TDM_TRY(frame_consts(ctx, cam_K, cam2base, num_scenes * num_frames, num_frames, im_height,
+                         im_width, dim_x, dim_y, dim_z, origins, 0.0f, 0.0f, 0.0f, voxel_size, s,
+                         &fc));
+    integrate_frames_kernel<<<grid, 64 * kRows, 0, s>>>(
+        fc, depth_ims, num_frames, im_height, im_width, dim_x, dim_y, dim_z, 0.0f, 0.0f, 0.0f,
+        voxel_size, trunc_margin, tsdf, weight, origins, (int)(grid.x / num_scenes));
     TDM_LAUNCHED();
     return TDM_OK;
 }
