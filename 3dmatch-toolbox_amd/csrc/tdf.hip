@@ -143,6 +143,69 @@ __global__ void scatter_bits_kernel(const float *__restrict__ pts, int n, int dx
     atomicOr(&bits[w], 1u << (gx & 31));
 }
 
+// The same scatter with a per-workgroup hash table in LDS: a workgroup ORs
+// the bits of its kScatThreads x kScatPts points per word in LDS, then issues
+// one device-scope atomicOr per distinct word.  Points cluster on surfaces
+// (the 200k-point benchmark fragment puts up to 1286 points in one word), and
+// device-scope atomics to one address serialise at the point all XCDs share;
+// on that fragment this kernel takes 17 us against 28-34 us for one atomic
+// per point.  Word indices must stay below 0xffffffff, the table's empty mark.
+constexpr int kScatThreads = 1024, kScatPts = 4;
+constexpr int kScatSlots = 8192;  // twice the keys of a workgroup: a probe always ends
+constexpr unsigned kScatEmpty = 0xffffffffu;
+
+__global__ __launch_bounds__(kScatThreads) void scatter_bits_dedup_kernel(
+    const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
+    float vs, unsigned *__restrict__ bits) {
+    __shared__ unsigned keys[kScatSlots];
+    __shared__ unsigned vals[kScatSlots];
+    for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads) {
+        keys[s] = kScatEmpty;
+        vals[s] = 0u;
+    }
+    const long rw = row_words(dx);
+    const long base = (long)blockIdx.x * kScatThreads * kScatPts + threadIdx.x;
+    // all loads first, then the decodes, then the inserts
+    float px[kScatPts], py[kScatPts], pz[kScatPts];
+#pragma unroll
+    for (int k = 0; k < kScatPts; ++k) {
+        const long i = min(base + (long)k * kScatThreads, (long)n - 1);
+        px[k] = pts[i * 3 + 0];
+        py[k] = pts[i * 3 + 1];
+        pz[k] = pts[i * 3 + 2];
+    }
+    unsigned w[kScatPts], bit[kScatPts];
+#pragma unroll
+    for (int k = 0; k < kScatPts; ++k) {
+        // point_voxel's arithmetic on the loaded coordinates
+        const float fx = roundf((px[k] - ox) / vs);
+        const float fy = roundf((py[k] - oy) / vs);
+        const float fz = roundf((pz[k] - oz) / vs);
+        const bool ok = base + (long)k * kScatThreads < n && fx >= 0.0f && fx < (float)dx &&
+                        fy >= 0.0f && fy < (float)dy && fz >= 0.0f && fz < (float)dz;
+        const int gx = ok ? (int)fx : 0, gy = ok ? (int)fy : 0, gz = ok ? (int)fz : 0;
+        w[k] = ok ? (unsigned)(((long)gz * dy + gy) * rw + (gx >> 5)) : kScatEmpty;
+        bit[k] = 1u << (gx & 31);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScatPts; ++k) {
+        if (w[k] == kScatEmpty) continue;
+        unsigned h = (w[k] * 2654435761u) & (kScatSlots - 1);  // Knuth's multiplicative hash
+        for (int probe = 0; probe < kScatSlots; ++probe) {
+            const unsigned prev = atomicCAS(&keys[h], kScatEmpty, w[k]);
+            if (prev == kScatEmpty || prev == w[k]) {
+                atomicOr(&vals[h], bit[k]);
+                break;
+            }
+            h = (h + 1) & (kScatSlots - 1);
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads)
+        if (keys[s] != kScatEmpty) atomicOr(&bits[keys[s]], vals[s]);
+}
+
 __global__ void scatter_occ_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
                                    float ox, float oy, float oz, float vs, float *__restrict__ occ) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -733,7 +796,14 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     size_t words = 0;
     unsigned char *flags = nullptr;
     TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], &bits, &words, &flags, s));
-    if (num_pts > 0) {
+    if (num_pts > 0 && words < (size_t)kScatEmpty) {
+        const long per_block = (long)kScatThreads * kScatPts;
+        scatter_bits_dedup_kernel<<<(unsigned)((num_pts + per_block - 1) / per_block),
+                                    kScatThreads, 0, s>>>(pts, num_pts, dims[0], dims[1],
+                                                          dims[2], origin[0], origin[1],
+                                                          origin[2], voxel_size, bits);
+        TDM_LAUNCHED();
+    } else if (num_pts > 0) {
         scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
             bits);
