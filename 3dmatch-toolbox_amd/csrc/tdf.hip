@@ -30,7 +30,9 @@ namespace {
 constexpr int kMaxR = 8;          // largest supported search radius (voxels)
 constexpr int kLutSize = 3 * kMaxR * kMaxR + 1;
 constexpr int kTX = 64, kTY = 16, kTZ = 16;
-constexpr int kTdfThreads = 256;
+constexpr int kTdfThreads = 256;       // a tile's workgroup in the batched form
+constexpr int kTdfThreadsOne = 1024;   // and for one cloud: a tile whose halo holds points
+                                       // is the critical path, so it gets 16 waves
 constexpr unsigned char kNone = 255;  // no occupied voxel within reach
 
 struct Lut {
@@ -116,6 +118,36 @@ __global__ void fill_u32_kernel(unsigned *__restrict__ p, size_t n, unsigned v) 
         p[i] = v;
 }
 
+// Zero n floats at p (16-byte stores over the aligned body, single floats for
+// the up to 3 before and after it) and nb bytes at the 16-byte aligned b, in
+// one launch.
+__global__ void clear_kernel(float *__restrict__ p, size_t n, size_t head, size_t n4,
+                             unsigned char *__restrict__ b, size_t nb) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nb16 = nb / 16;
+    float4 *__restrict__ body = reinterpret_cast<float4 *>(p + head);
+    uint4 *__restrict__ bbody = reinterpret_cast<uint4 *>(b);
+    for (size_t i = t; i < n4 + nb16; i += (size_t)gridDim.x * blockDim.x) {
+        if (i < n4) body[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else bbody[i - n4] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (t < head) p[t] = 0.0f;
+    const size_t tail0 = head + 4 * n4;
+    if (t < n - tail0) p[tail0 + t] = 0.0f;
+    if (t < nb - 16 * nb16) b[16 * nb16 + t] = 0u;
+}
+
+// p and b may be null (nothing is cleared there).
+void clear_regions(float *p, size_t n, unsigned char *b, size_t nb, hipStream_t s) {
+    if (!p) n = 0;
+    if (!b) nb = 0;
+    size_t head = p ? ((16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15) / sizeof(float) : 0;
+    if (head > n) head = n;
+    const size_t n4 = (n - head) / 4;
+    const unsigned blocks = (unsigned)std::min<size_t>((n4 + nb / 16) / 256 + 1, 16384);
+    clear_kernel<<<blocks, 256, 0, s>>>(p, n, head, n4, b, nb);
+}
+
 // core/demo.cu:129-134: voxel = round((p - origin)/vs) per axis.
 __device__ inline bool point_voxel(const float *__restrict__ pts, long i, int dx, int dy, int dz,
                                    float ox, float oy, float oz, float vs, int &gx, int &gy,
@@ -132,15 +164,23 @@ __device__ inline bool point_voxel(const float *__restrict__ pts, long i, int dx
     return true;
 }
 
+// Flag byte of the 64x16x16 tile holding voxel (gx, gy, gz): set to 1 by
+// every point in it (the same value from every writer, so plain stores do).
+__device__ inline long tile_of(int gx, int gy, int gz, int dx, int dy) {
+    const int ntx = (dx + kTX - 1) / kTX, nty = (dy + kTY - 1) / kTY;
+    return ((long)(gz / kTZ) * nty + gy / kTY) * ntx + gx / kTX;
+}
+
 __global__ void scatter_bits_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
                                     float ox, float oy, float oz, float vs,
-                                    unsigned *__restrict__ bits) {
+                                    unsigned *__restrict__ bits, unsigned char *__restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int gx, gy, gz;
     if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
     long w = ((long)gz * dy + gy) * row_words(dx) + (gx >> 5);
     atomicOr(&bits[w], 1u << (gx & 31));
+    flags[tile_of(gx, gy, gz, dx, dy)] = 1;
 }
 
 // The same scatter with a per-workgroup hash table in LDS: a workgroup ORs
@@ -156,7 +196,7 @@ constexpr unsigned kScatEmpty = 0xffffffffu;
 
 __global__ __launch_bounds__(kScatThreads) void scatter_bits_dedup_kernel(
     const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
-    float vs, unsigned *__restrict__ bits) {
+    float vs, unsigned *__restrict__ bits, unsigned char *__restrict__ flags) {
     __shared__ unsigned keys[kScatSlots];
     __shared__ unsigned vals[kScatSlots];
     for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads) {
@@ -186,6 +226,7 @@ __global__ __launch_bounds__(kScatThreads) void scatter_bits_dedup_kernel(
         const int gx = ok ? (int)fx : 0, gy = ok ? (int)fy : 0, gz = ok ? (int)fz : 0;
         w[k] = ok ? (unsigned)(((long)gz * dy + gy) * rw + (gx >> 5)) : kScatEmpty;
         bit[k] = 1u << (gx & 31);
+        if (ok) flags[tile_of(gx, gy, gz, dx, dy)] = 1;
     }
     __syncthreads();
 #pragma unroll
@@ -270,10 +311,14 @@ __device__ inline unsigned char xpass(unsigned w0, unsigned w1, unsigned w2, uns
 }
 
 // Truncated squared-distance transform of the 64x16x16 tile at (X0, Y0, Z0).
-// With `flags` (one byte per tile: any occupied voxel inside it), a tile
-// whose 27 neighbouring tiles -- which contain its halo, R < 16 -- hold no
-// occupied voxel writes zeros without loading the halo.
-template <int R>
+// With `flags` (one byte per tile: any occupied voxel inside it) the grid has
+// been zeroed beforehand with full-line stores: a tile whose 27 neighbouring
+// tiles -- which contain its halo, R < 16 -- hold no occupied voxel, or whose
+// halo turns out empty, writes nothing.  (A tile row is 256 bytes at a row
+// pitch of 4*dx bytes, generally not line-aligned; storing a zero grid that
+// way runs at well under half the rate of a contiguous fill.)  Without
+// `flags` every tile writes all its voxels.
+template <int R, int NT>
 __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits, int dx, int dy,
                                 int dz, const Lut &lut, float *__restrict__ tdf, int X0, int Y0,
                                 int Z0, const unsigned char *__restrict__ flags) {
@@ -297,20 +342,13 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
             if (tx >= 0 && tx < ntx && ty >= 0 && ty < nty && tz >= 0 && tz < ntz)
                 near = flags[((long)tz * nty + ty) * ntx + tx] != 0;
         }
-        if (__syncthreads_or(near) == 0) {
-            const int x = X0 + lane;
-            for (int row = wave; row < kTY * kTZ; row += kTdfThreads / 64) {
-                const int gy = Y0 + row % kTY, gz = Z0 + row / kTY;
-                if (gy < dy && gz < dz && x < dx) tdf[((long)gz * dy + gy) * dx + x] = 0.0f;
-            }
-            return;
-        }
+        if (__syncthreads_or(near) == 0) return;
     }
 
-    for (int i = threadIdx.x; i < kLutSize; i += kTdfThreads) slut[i] = lut.v[i];
+    for (int i = threadIdx.x; i < kLutSize; i += NT) slut[i] = lut.v[i];
     // All halo words at once (independent loads, so their latencies overlap).
     unsigned seen = 0;
-    for (int i = threadIdx.x; i < HZ * HY * 4; i += kTdfThreads) {
+    for (int i = threadIdx.x; i < HZ * HY * 4; i += NT) {
         const int row = i >> 2, wi = (int)(i & 3);
         const int y = Y0 + row % HY - R, z = Z0 + row / HY - R;
         const long w = wbase - 1 + wi;
@@ -321,11 +359,12 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
         seen |= v;
     }
     const bool empty = __syncthreads_or(seen != 0u) == 0;
+    if (empty && flags) return;
 
     if (!empty) {
         // Passes x and y over the halo z-slab zz: pass x of its HY rows in
         // registers, then min over |d| <= R of f(y + d) + d^2 for the tile's y.
-        for (int zz = wave; zz < HZ; zz += kTdfThreads / 64) {
+        for (int zz = wave; zz < HZ; zz += NT / 64) {
             unsigned char f[HY];
 #pragma unroll
             for (int yy = 0; yy < HY; ++yy) {
@@ -347,7 +386,7 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
 
     // Pass z and the table lookup; write the tile.
     const int x = X0 + lane;
-    for (int row = wave; row < kTY * kTZ; row += kTdfThreads / 64) {
+    for (int row = wave; row < kTY * kTZ; row += NT / 64) {
         const int y = row % kTY, zl = row / kTY;
         const int gy = Y0 + y, gz = Z0 + zl;
         if (gy >= dy || gz >= dz || x >= dx) continue;
@@ -364,12 +403,12 @@ __device__ inline void tdf_tile(TileLds<R> &L, const unsigned *__restrict__ bits
 }
 
 template <int R>
-__global__ __launch_bounds__(kTdfThreads) void tdf_tile_kernel(const unsigned *__restrict__ bits,
+__global__ __launch_bounds__(kTdfThreadsOne) void tdf_tile_kernel(const unsigned *__restrict__ bits,
                                                                int dx, int dy, int dz, Lut lut,
                                                                float *__restrict__ tdf,
                                                                const unsigned char *__restrict__ flags) {
     __shared__ TileLds<R> L;
-    tdf_tile<R>(L, bits, dx, dy, dz, lut, tdf, blockIdx.x * kTX, blockIdx.y * kTY,
+    tdf_tile<R, kTdfThreadsOne>(L, bits, dx, dy, dz, lut, tdf, blockIdx.x * kTX, blockIdx.y * kTY,
                 blockIdx.z * kTZ, flags);
 }
 
@@ -450,7 +489,7 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_batch_kernel(
     const int tx = (int)(lt % cd.tiles[0]);
     const int ty = (int)((lt / cd.tiles[0]) % cd.tiles[1]);
     const int tz = (int)(lt / ((long)cd.tiles[0] * cd.tiles[1]));
-    tdf_tile<R>(L, bits + cd.word_off, cd.dims[0], cd.dims[1], cd.dims[2], lut, cd.tdf, tx * kTX,
+    tdf_tile<R, kTdfThreads>(L, bits + cd.word_off, cd.dims[0], cd.dims[1], cd.dims[2], lut, cd.tdf, tx * kTX,
                 ty * kTY, tz * kTZ, nullptr);
 }
 
@@ -638,21 +677,25 @@ template <int R>
 void launch_tile(const unsigned *bits, int dx, int dy, int dz, const Lut &lut, float *tdf,
                  const unsigned char *flags, hipStream_t s) {
     dim3 grid((dx + kTX - 1) / kTX, (dy + kTY - 1) / kTY, (dz + kTZ - 1) / kTZ);
-    tdf_tile_kernel<R><<<grid, kTdfThreads, 0, s>>>(bits, dx, dy, dz, lut, tdf, flags);
+    tdf_tile_kernel<R><<<grid, kTdfThreadsOne, 0, s>>>(bits, dx, dy, dz, lut, tdf, flags);
 }
 
 long num_tiles(int dx, int dy, int dz) {
     return (long)((dx + kTX - 1) / kTX) * ((dy + kTY - 1) / kTY) * ((dz + kTZ - 1) / kTZ);
 }
 
-// TDF from a bit grid already in scratch; `flags` (num_tiles bytes of
-// scratch) receives the per-tile occupancy.
+// TDF from a bit grid in scratch into a grid that bits_scratch zeroed (the
+// tile kernel writes only tiles whose halo holds a point).  `flags` (num_tiles
+// bytes of scratch) holds the per-tile occupancy when flags_ready, else it is
+// computed here from the bits.
 int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decode, float *tdf,
-                  unsigned char *flags, hipStream_t s) {
+                  unsigned char *flags, bool flags_ready, hipStream_t s) {
     Lut lut = make_lut(r);
-    const long nt = num_tiles(dx, dy, dz);
-    tile_flags_kernel<<<(unsigned)((nt + 3) / 4), 256, 0, s>>>(bits, dx, dy, dz, flags);
-    TDM_LAUNCHED();
+    if (!flags_ready) {
+        const long nt = num_tiles(dx, dy, dz);
+        tile_flags_kernel<<<(unsigned)((nt + 3) / 4), 256, 0, s>>>(bits, dx, dy, dz, flags);
+        TDM_LAUNCHED();
+    }
     switch (r) {
     case 1: launch_tile<1>(bits, dx, dy, dz, lut, tdf, flags, s); break;
     case 2: launch_tile<2>(bits, dx, dy, dz, lut, tdf, flags, s); break;
@@ -674,16 +717,17 @@ int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decod
     return TDM_OK;
 }
 
-// Bit grid (cleared) and, after it, the per-tile flags, in scratch slot 0.
-int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, unsigned **bits, size_t *words,
-                 unsigned char **flags, hipStream_t s) {
+// Bit grid and, after it, the per-tile flags, in scratch slot 0; one launch
+// clears both and zeroes the output grid tdf (dx*dy*dz floats; none if null).
+int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, float *tdf, unsigned **bits,
+                 size_t *words, unsigned char **flags, hipStream_t s) {
     *words = (size_t)row_words(dx) * dy * dz;
+    const size_t bytes = *words * sizeof(unsigned) + num_tiles(dx, dy, dz);
     void *p = nullptr;
-    TDM_TRY(tdm::ctx_scratch(ctx, 0, *words * sizeof(unsigned) + num_tiles(dx, dy, dz), &p, s));
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &p, s));
     *bits = static_cast<unsigned *>(p);
     *flags = reinterpret_cast<unsigned char *>(*bits + *words);
-    unsigned blocks = (unsigned)std::min<size_t>(tdm::blocks_for(*words, 256), 16384);
-    fill_u32_kernel<<<blocks, 256, 0, s>>>(*bits, *words, 0u);
+    clear_regions(tdf, (size_t)dx * dy * dz, static_cast<unsigned char *>(p), bytes, s);
     TDM_LAUNCHED();
     return TDM_OK;
 }
@@ -769,13 +813,16 @@ int tdm_compute_tdf(tdm_ctx *ctx, const float *occ, float *tdf, int dim_x, int d
     unsigned *bits = nullptr;
     size_t words = 0;
     unsigned char *flags = nullptr;
-    TDM_TRY(bits_scratch(ctx, dim_x, dim_y, dim_z, &bits, &words, &flags, s));
+    TDM_TRY(bits_scratch(ctx, dim_x, dim_y, dim_z, nullptr, &bits, &words, &flags, s));
     const long rows = (long)dim_y * dim_z;
     const long pairs = (row_words(dim_x) + 1) / 2;
     const long waves = rows * pairs;
     pack_bits_kernel<<<tdm::blocks_for(waves * 64, 256), 256, 0, s>>>(occ, dim_x, rows, bits);
     TDM_LAUNCHED();
-    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, flags, s);
+    // zeroed after the packing, so occ and tdf may be one buffer
+    clear_regions(tdf, (size_t)dim_x * dim_y * dim_z, nullptr, 0, s);
+    TDM_LAUNCHED();
+    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, flags, false, s);
 }
 
 int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
@@ -795,21 +842,21 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     unsigned *bits = nullptr;
     size_t words = 0;
     unsigned char *flags = nullptr;
-    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], &bits, &words, &flags, s));
+    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], tdf, &bits, &words, &flags, s));
     if (num_pts > 0 && words < (size_t)kScatEmpty) {
         const long per_block = (long)kScatThreads * kScatPts;
         scatter_bits_dedup_kernel<<<(unsigned)((num_pts + per_block - 1) / per_block),
                                     kScatThreads, 0, s>>>(pts, num_pts, dims[0], dims[1],
                                                           dims[2], origin[0], origin[1],
-                                                          origin[2], voxel_size, bits);
+                                                          origin[2], voxel_size, bits, flags);
         TDM_LAUNCHED();
     } else if (num_pts > 0) {
         scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
-            bits);
+            bits, flags);
         TDM_LAUNCHED();
     }
-    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, flags, s);
+    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, flags, true, s);
 }
 
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
