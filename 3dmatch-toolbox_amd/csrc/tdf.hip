@@ -118,34 +118,46 @@ __global__ void fill_u32_kernel(unsigned *__restrict__ p, size_t n, unsigned v) 
         p[i] = v;
 }
 
-// Zero n floats at p (16-byte stores over the aligned body, single floats for
-// the up to 3 before and after it) and nb bytes at the 16-byte aligned b, in
-// one launch.
-__global__ void clear_kernel(float *__restrict__ p, size_t n, size_t head, size_t n4,
-                             unsigned char *__restrict__ b, size_t nb) {
+// Zeroing of n floats at p: 16-byte stores over the aligned body (n4 float4
+// from p + head), single floats for the up to 3 before and after it.
+struct ZeroJob {
+    float *p;
+    size_t n, head, n4;
+};
+
+ZeroJob zero_job(float *p, size_t n) {  // p may be null (nothing to zero)
+    ZeroJob z{p, p ? n : 0, 0, 0};
+    if (p) z.head = std::min<size_t>(((16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15) / 4, n);
+    z.n4 = (z.n - z.head) / 4;
+    return z;
+}
+
+// Thread t of `stride` threads.
+__device__ inline void zero_floats_part(const ZeroJob &z, size_t t, size_t stride) {
+    float4 *__restrict__ body = reinterpret_cast<float4 *>(z.p + z.head);
+    for (size_t i = t; i < z.n4; i += stride) body[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t < z.head) z.p[t] = 0.0f;
+    const size_t tail0 = z.head + 4 * z.n4;
+    if (t < z.n - tail0) z.p[tail0 + t] = 0.0f;
+}
+
+// The floats of z and nb bytes at the 16-byte aligned b, in one launch.
+__global__ void clear_kernel(ZeroJob z, unsigned char *__restrict__ b, size_t nb) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    zero_floats_part(z, t, stride);
     const size_t nb16 = nb / 16;
-    float4 *__restrict__ body = reinterpret_cast<float4 *>(p + head);
     uint4 *__restrict__ bbody = reinterpret_cast<uint4 *>(b);
-    for (size_t i = t; i < n4 + nb16; i += (size_t)gridDim.x * blockDim.x) {
-        if (i < n4) body[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        else bbody[i - n4] = make_uint4(0u, 0u, 0u, 0u);
-    }
-    if (t < head) p[t] = 0.0f;
-    const size_t tail0 = head + 4 * n4;
-    if (t < n - tail0) p[tail0 + t] = 0.0f;
+    for (size_t i = t; i < nb16; i += stride) bbody[i] = make_uint4(0u, 0u, 0u, 0u);
     if (t < nb - 16 * nb16) b[16 * nb16 + t] = 0u;
 }
 
 // p and b may be null (nothing is cleared there).
 void clear_regions(float *p, size_t n, unsigned char *b, size_t nb, hipStream_t s) {
-    if (!p) n = 0;
+    const ZeroJob z = zero_job(p, n);
     if (!b) nb = 0;
-    size_t head = p ? ((16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15) / sizeof(float) : 0;
-    if (head > n) head = n;
-    const size_t n4 = (n - head) / 4;
-    const unsigned blocks = (unsigned)std::min<size_t>((n4 + nb / 16) / 256 + 1, 16384);
-    clear_kernel<<<blocks, 256, 0, s>>>(p, n, head, n4, b, nb);
+    const unsigned blocks = (unsigned)std::min<size_t>((z.n4 + nb / 16) / 256 + 1, 16384);
+    clear_kernel<<<blocks, 256, 0, s>>>(z, b, nb);
 }
 
 // core/demo.cu:129-134: voxel = round((p - origin)/vs) per axis.
@@ -194,11 +206,21 @@ constexpr int kScatThreads = 1024, kScatPts = 4;
 constexpr int kScatSlots = 8192;  // twice the keys of a workgroup: a probe always ends
 constexpr unsigned kScatEmpty = 0xffffffffu;
 
+//
+// Workgroups from `nscat` on zero the output grid (z) instead: that store
+// stream needs the memory bandwidth the scatter leaves idle, and sharing the
+// launch runs the two side by side.
 __global__ __launch_bounds__(kScatThreads) void scatter_bits_dedup_kernel(
     const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
-    float vs, unsigned *__restrict__ bits, unsigned char *__restrict__ flags) {
+    float vs, unsigned *__restrict__ bits, unsigned char *__restrict__ flags, ZeroJob z,
+    unsigned nscat) {
     __shared__ unsigned keys[kScatSlots];
     __shared__ unsigned vals[kScatSlots];
+    if (blockIdx.x >= nscat) {
+        zero_floats_part(z, (size_t)(blockIdx.x - nscat) * kScatThreads + threadIdx.x,
+                         (size_t)(gridDim.x - nscat) * kScatThreads);
+        return;
+    }
     for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads) {
         keys[s] = kScatEmpty;
         vals[s] = 0u;
@@ -842,13 +864,19 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     unsigned *bits = nullptr;
     size_t words = 0;
     unsigned char *flags = nullptr;
-    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], tdf, &bits, &words, &flags, s));
-    if (num_pts > 0 && words < (size_t)kScatEmpty) {
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    const bool dedup = num_pts > 0 && (size_t)row_words(dims[0]) * dims[1] * dims[2] < kScatEmpty;
+    // the deduplicating scatter zeroes the output grid in its own launch
+    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], dedup ? nullptr : tdf, &bits, &words,
+                         &flags, s));
+    if (dedup) {
         const long per_block = (long)kScatThreads * kScatPts;
-        scatter_bits_dedup_kernel<<<(unsigned)((num_pts + per_block - 1) / per_block),
-                                    kScatThreads, 0, s>>>(pts, num_pts, dims[0], dims[1],
-                                                          dims[2], origin[0], origin[1],
-                                                          origin[2], voxel_size, bits, flags);
+        const unsigned nscat = (unsigned)((num_pts + per_block - 1) / per_block);
+        const ZeroJob z = zero_job(tdf, nvox);
+        const unsigned nfill = (unsigned)std::min<size_t>(z.n4 / kScatThreads + 1, 512);
+        scatter_bits_dedup_kernel<<<nscat + nfill, kScatThreads, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            bits, flags, z, nscat);
         TDM_LAUNCHED();
     } else if (num_pts > 0) {
         scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
