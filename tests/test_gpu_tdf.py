@@ -91,6 +91,53 @@ def test_empty_and_single_point(torch_gpu):
     assert np.array_equal(tdf.cpu().numpy(), oracle.compute_tdf(occ, vs, trunc))
 
 
+def test_tdf_from_points_edges(torch_gpu):
+    """Clustered points (thousands per bit word, more than a scatter
+    workgroup's 4096), points outside the grid, an empty cloud and an output
+    grid that starts 4 bytes past a 16-byte boundary, each on a grid that
+    first held a larger cloud's result; every voxel is compared."""
+    rng = np.random.default_rng(7)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    dims, origin = (133, 37, 29), np.array([-0.3, -0.1, 0.2], np.float32)
+    n = dims[0] * dims[1] * dims[2]
+    lo = origin + np.float32(3) * vs
+    span = np.array([0.9, 0.2, 0.15], np.float32)
+    sparse = (lo + rng.random((9000, 3)) * span).astype(np.float32)
+    cluster = (lo + np.float32(0.5) * span + rng.random((20000, 3)) * np.float32(0.05)).astype(np.float32)
+    outside = np.array([[5.0, 5.0, 5.0], [-5.0, 0.0, 0.3], [0.0, 0.0, 9.0]], np.float32)
+    cases = {
+        "mixed": np.concatenate([cluster[:7000], sparse, outside, cluster[7000:]]),
+        "cluster": cluster,
+        "outside": outside,
+        "empty": np.zeros((0, 3), np.float32),
+    }
+    backing = torch_gpu.empty(n + 1, dtype=torch_gpu.float32, device="cuda:0")
+    tdf = backing[1:]  # 4 bytes past the allocation's 256-byte alignment
+    for name, pts in cases.items():
+        print(f"case tdf_from_points edges: {name} ({len(pts)} points)", flush=True)
+        backing.fill_(7.0)
+        tdm.tdf_from_points(_dev(torch_gpu, pts), dims, origin, vs, trunc, tdf)
+        got = tdf.cpu().numpy().reshape(dims[2], dims[1], dims[0])
+        occ = oracle.occupancy(pts, dims, origin, vs)
+        ref = oracle.compute_tdf(occ, vs, trunc)
+        assert backing[0].item() == 7.0, "write before the grid"
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), \
+            f"{name}: {np.count_nonzero(got != ref)} voxels differ"
+
+
+def test_compute_tdf_in_place(torch_gpu):
+    """occ and tdf may be one buffer: the bits are packed before the output
+    is written."""
+    rng = np.random.default_rng(3)
+    dims = (100, 40, 33)
+    occ = (rng.random((dims[2], dims[1], dims[0])) < 0.003).astype(np.float32)
+    vs, trunc = np.float32(0.01), np.float32(0.05)
+    buf = _dev(torch_gpu, occ)
+    tdm.compute_tdf(buf, buf, dims, vs, trunc)
+    ref = oracle.compute_tdf(occ, vs, trunc)
+    assert np.array_equal(buf.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
 def test_reference_decode_beyond_2_24(torch_gpu):
     """On a grid of more than 2^24 voxels the reference's float32 decode of
     voxel_idx (core/demo.cu:32-34) is wrong for part of the voxels; the
