@@ -699,8 +699,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(each box has a single MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the workloads are sized for one MI355X); run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
