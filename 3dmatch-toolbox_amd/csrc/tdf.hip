@@ -554,6 +554,50 @@ __global__ void tdf_refdecode_kernel(const unsigned *__restrict__ bits, int dx, 
     }
 }
 
+// Reference decode for one grid, from the exact TDF in a separate buffer.
+// Voxel v searches around its float32 decode (x', y', z') but takes its
+// occupancy at v (core/demo.cu:32-56).  Where the decode is v itself the value
+// is exact[v]; where it is another voxel j inside the grid the search is the
+// one that gives j its exact TDF (an occupied j is within reach at distance
+// 0), so v gets 1 if occupied, else exact[j]; decodes outside the grid are
+// searched here as in tdf_refdecode_kernel.
+__global__ void tdf_refgather_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
+                                     int r, Lut lut, const float *__restrict__ exact,
+                                     float *__restrict__ tdf) {
+    // one workgroup per 256 x of a row: the true coordinates need only two
+    // (uniform) divisions per workgroup
+    const long rw = row_words(dx);
+    const unsigned nbx = (unsigned)(dx + 255) / 256;
+    const unsigned row = blockIdx.x / nbx;
+    const int tx = (int)(blockIdx.x - row * nbx) * 256 + (int)threadIdx.x;
+    const int ty = (int)(row % (unsigned)dy), tz = (int)(row / (unsigned)dy);
+    if (tx >= dx) return;
+    const long v = ((long)tz * dy + ty) * dx + tx;
+    {
+        int x, y, z;
+        float_decode(v, dx, dy, x, y, z);
+        float val;
+        if (x == tx && y == ty && z == tz) {
+            val = exact[v];
+        } else if (bit_at(bits, rw, dy, tx, ty, tz)) {
+            val = 1.0f;
+        } else if (x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz) {
+            val = exact[((long)z * dy + y) * dx + x];
+        } else {
+            int best = kLutSize;
+            for (int ix = max(0, x - r); ix < min(dx, x + r + 1); ++ix)
+                for (int iy = max(0, y - r); iy < min(dy, y + r + 1); ++iy)
+                    for (int iz = max(0, z - r); iz < min(dz, z + r + 1); ++iz)
+                        if (bit_at(bits, rw, dy, ix, iy, iz)) {
+                            int d2 = (x - ix) * (x - ix) + (y - iy) * (y - iy) + (z - iz) * (z - iz);
+                            best = min(best, d2);
+                        }
+            val = best < kLutSize ? lut.v[best] : 0.0f;
+        }
+        tdf[v] = val;
+    }
+}
+
 // ---- patch gather --------------------------------------------------------------------
 
 __global__ void extract_patches_kernel(const float *__restrict__ tdf, int dx, int dy, int dz,
@@ -710,8 +754,10 @@ long num_tiles(int dx, int dy, int dz) {
 // tile kernel writes only tiles whose halo holds a point).  `flags` (num_tiles
 // bytes of scratch) holds the per-tile occupancy when flags_ready, else it is
 // computed here from the bits.
-int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decode, float *tdf,
-                  unsigned char *flags, bool flags_ready, hipStream_t s) {
+// The exact TDF goes to `work` (already zeroed); in reference decode `work`
+// is a separate buffer and tdf receives the gathered result, else work == tdf.
+int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decode, float *work,
+                  float *tdf, unsigned char *flags, bool flags_ready, hipStream_t s) {
     Lut lut = make_lut(r);
     if (!flags_ready) {
         const long nt = num_tiles(dx, dy, dz);
@@ -719,23 +765,35 @@ int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decod
         TDM_LAUNCHED();
     }
     switch (r) {
-    case 1: launch_tile<1>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 2: launch_tile<2>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 3: launch_tile<3>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 4: launch_tile<4>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 5: launch_tile<5>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 6: launch_tile<6>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 7: launch_tile<7>(bits, dx, dy, dz, lut, tdf, flags, s); break;
-    case 8: launch_tile<8>(bits, dx, dy, dz, lut, tdf, flags, s); break;
+    case 1: launch_tile<1>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 2: launch_tile<2>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 3: launch_tile<3>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 4: launch_tile<4>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 5: launch_tile<5>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 6: launch_tile<6>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 7: launch_tile<7>(bits, dx, dy, dz, lut, work, flags, s); break;
+    case 8: launch_tile<8>(bits, dx, dy, dz, lut, work, flags, s); break;
     default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", r);
     }
     TDM_LAUNCHED();
     if (decode == TDM_DECODE_REFERENCE) {
-        long n = (long)dx * dy * dz;
-        unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
-        tdf_refdecode_kernel<<<blocks, 256, 0, s>>>(bits, dx, dy, dz, r, lut, tdf);
+        // at most (dx/256 + 1) * dy * dz <= 2^31 workgroups
+        const unsigned blocks = (unsigned)((long)((dx + 255) / 256) * dy * dz);
+        tdf_refgather_kernel<<<blocks, 256, 0, s>>>(bits, dx, dy, dz, r, lut, work, tdf);
         TDM_LAUNCHED();
     }
+    return TDM_OK;
+}
+
+// Where the exact TDF is computed: tdf itself, or for reference decode a
+// dx*dy*dz-float buffer in the context's scratch slot 2 that the gather then
+// reads.
+int tdf_work(tdm_ctx *ctx, int decode, size_t nvox, float *tdf, hipStream_t s, float **work) {
+    *work = tdf;
+    if (decode != TDM_DECODE_REFERENCE) return TDM_OK;
+    void *p = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 2, sizeof(float) * nvox, &p, s));
+    *work = static_cast<float *>(p);
     return TDM_OK;
 }
 
@@ -842,9 +900,12 @@ int tdm_compute_tdf(tdm_ctx *ctx, const float *occ, float *tdf, int dim_x, int d
     pack_bits_kernel<<<tdm::blocks_for(waves * 64, 256), 256, 0, s>>>(occ, dim_x, rows, bits);
     TDM_LAUNCHED();
     // zeroed after the packing, so occ and tdf may be one buffer
-    clear_regions(tdf, (size_t)dim_x * dim_y * dim_z, nullptr, 0, s);
+    const size_t nvox = (size_t)dim_x * dim_y * dim_z;
+    float *work = nullptr;
+    TDM_TRY(tdf_work(ctx, decode, nvox, tdf, s, &work));
+    clear_regions(work, nvox, nullptr, 0, s);
     TDM_LAUNCHED();
-    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, tdf, flags, false, s);
+    return tdf_from_bits(bits, dim_x, dim_y, dim_z, r, decode, work, tdf, flags, false, s);
 }
 
 int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
@@ -865,14 +926,16 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     size_t words = 0;
     unsigned char *flags = nullptr;
     const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    float *work = nullptr;
+    TDM_TRY(tdf_work(ctx, decode, nvox, tdf, s, &work));
     const bool dedup = num_pts > 0 && (size_t)row_words(dims[0]) * dims[1] * dims[2] < kScatEmpty;
     // the deduplicating scatter zeroes the output grid in its own launch
-    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], dedup ? nullptr : tdf, &bits, &words,
+    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], dedup ? nullptr : work, &bits, &words,
                          &flags, s));
     if (dedup) {
         const long per_block = (long)kScatThreads * kScatPts;
         const unsigned nscat = (unsigned)((num_pts + per_block - 1) / per_block);
-        const ZeroJob z = zero_job(tdf, nvox);
+        const ZeroJob z = zero_job(work, nvox);
         const unsigned nfill = (unsigned)std::min<size_t>(z.n4 / kScatThreads + 1, 512);
         scatter_bits_dedup_kernel<<<nscat + nfill, kScatThreads, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
@@ -884,7 +947,7 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
             bits, flags);
         TDM_LAUNCHED();
     }
-    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, tdf, flags, true, s);
+    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, work, tdf, flags, true, s);
 }
 
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
