@@ -645,28 +645,51 @@ __global__ void acc_count_kernel(const float *__restrict__ pts, int n, int dx, i
     atomicAdd(&counts[acc_tile_of(gx, gy, gz, tx, ty)], 1);
 }
 
-// Exclusive scan of n counts (one workgroup); offsets[n] = total.
-__global__ __launch_bounds__(1024) void acc_scan_kernel(const int *__restrict__ counts, long n,
-                                                        int *__restrict__ offsets) {
-    __shared__ int part[1024];
-    const long per = (n + 1023) / 1024;
-    const long b0 = min(n, (long)threadIdx.x * per), b1 = min(n, b0 + per);
-    int sum = 0;
-    for (long b = b0; b < b1; ++b) sum += counts[b];
-    part[threadIdx.x] = sum;
+// Exclusive scan of n counts in two launches: per-1024 block sums, then
+// each block scans its own counts on top of the sum of the blocks before it.
+// offsets[n] = total.
+__device__ inline int block_sum_1024(int v, int *red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    int tot = 0;
+    for (int w = 0; w < 16; ++w) tot += red[w];
+    return tot;
+}
+
+__global__ __launch_bounds__(1024) void acc_blocksum_kernel(const int *__restrict__ counts, long n,
+                                                            int *__restrict__ bsum) {
+    __shared__ int red[16];
+    const long i = (long)blockIdx.x * 1024 + threadIdx.x;
+    const int tot = block_sum_1024(i < n ? counts[i] : 0, red);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void acc_offsets_kernel(const int *__restrict__ counts, long n,
+                                                           const int *__restrict__ bsum,
+                                                           int *__restrict__ offsets) {
+    __shared__ int red[16];
+    __shared__ int wsum[16];
+    int part = 0;
+    for (unsigned b = threadIdx.x; b < blockIdx.x; b += 1024) part += bsum[b];
+    const int base = block_sum_1024(part, red);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long i = (long)blockIdx.x * 1024 + threadIdx.x;
+    const int v = i < n ? counts[i] : 0;
+    int inc = v;  // inclusive scan within the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off);
+        if (lane >= off) inc += u;
     }
-    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
-    for (long b = b0; b < b1; ++b) {
-        offsets[b] = run;
-        run += counts[b];
-    }
-    if (threadIdx.x == 1023) offsets[n] = part[1023];
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    const int excl = base + before + inc - v;
+    if (i < n) offsets[i] = excl;
+    if (i == n - 1) offsets[n] = excl + v;
 }
 
 __global__ void acc_fill_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
@@ -684,7 +707,14 @@ __global__ void acc_fill_kernel(const float *__restrict__ pts, int n, int dx, in
 
 // One 8x8x8 tile per workgroup of 512 threads.  Points of the tiles within
 // `reach` tiles (reach*8 voxels >= the truncation radius plus the half voxel
-// a point may sit off its voxel) are staged through LDS 512 at a time.
+// a point may sit off its voxel) are staged through LDS 512 at a time, and
+// only those that can be within the truncation distance of one of the
+// tile's voxel centres are kept: the result is 1 - min(d, trunc)/trunc of the
+// nearest point, so a point at distance >= trunc from every centre changes
+// nothing.  The test uses the box of the centres, computed with the same
+// float operations as the centres (so it contains them); float rounding is
+// monotone, so every centre's d^2 is at least the box's, and the threshold
+// trunc^2 (1 + 2^-20) clears the rounding of trunc^2 itself.
 __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
                                                       const int *__restrict__ offsets, int dx,
                                                       int dy, int dz, int tx, int ty, int tz,
@@ -692,12 +722,18 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
                                                       float vs, float trunc,
                                                       float *__restrict__ tdf) {
     __shared__ float4 sp[512];
+    __shared__ int wave_n[8];
     const int t = blockIdx.x;
     const int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
     const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
     const int x = bx * kAT + lx, y = by * kAT + ly, z = bz * kAT + lz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // voxel centre (origin is the centre of voxel 0)
     const float cx = ox + (float)x * vs, cy = oy + (float)y * vs, cz = oz + (float)z * vs;
+    const float lox = ox + (float)(bx * kAT) * vs, hix = ox + (float)(bx * kAT + kAT - 1) * vs;
+    const float loy = oy + (float)(by * kAT) * vs, hiy = oy + (float)(by * kAT + kAT - 1) * vs;
+    const float loz = oz + (float)(bz * kAT) * vs, hiz = oz + (float)(bz * kAT + kAT - 1) * vs;
+    const float lim = trunc * trunc * (1.0f + 0x1p-20f);
     float best = __builtin_huge_valf();
     for (int nz = max(0, bz - reach); nz <= min(tz - 1, bz + reach); ++nz)
         for (int ny = max(0, by - reach); ny <= min(ty - 1, by + reach); ++ny)
@@ -706,10 +742,28 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
                 const int p0 = offsets[nt], p1 = offsets[nt + 1];
                 for (int c0 = p0; c0 < p1; c0 += 512) {
                     const int cn = min(512, p1 - c0);
+                    bool keep = false;
+                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if ((int)threadIdx.x < cn) {
+                        q = binned[c0 + threadIdx.x];
+                        const float ex = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.0f);
+                        const float ey = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.0f);
+                        const float ez = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.0f);
+                        keep = ex * ex + ey * ey + ez * ez <= lim;
+                    }
+                    const unsigned long long m = __ballot(keep);
+                    __syncthreads();  // the previous chunk's reads of sp are done
+                    if (lane == 0) wave_n[wave] = __popcll(m);
                     __syncthreads();
-                    if ((int)threadIdx.x < cn) sp[threadIdx.x] = binned[c0 + threadIdx.x];
+                    int pos = __popcll(m & ((1ull << lane) - 1ull)), kept = 0;
+                    for (int w = 0; w < 8; ++w) {
+                        const int c = wave_n[w];
+                        pos += w < wave ? c : 0;
+                        kept += c;
+                    }
+                    if (keep) sp[pos] = q;
                     __syncthreads();
-                    for (int k = 0; k < cn; ++k) {
+                    for (int k = 0; k < kept; ++k) {
                         const float4 q = sp[k];
                         const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
                         const float d2 = ex * ex + ey * ey + ez * ez;
@@ -1002,13 +1056,15 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     const int reach = (int)std::ceil(reach_vox / kAT);
     TDM_CHECK(reach <= 4, TDM_ERR_UNSUPPORTED, "truncation of %g voxels is too wide",
               (double)trunc_margin / voxel_size);
-    const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1) + 16 + sizeof(float4) * (size_t)num_pts;
+    const long nb = (nt + 1023) / 1024;  // scan blocks
+    const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1 + nb) + 16 + sizeof(float4) * (size_t)num_pts;
     void *scratch = nullptr;
     TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &scratch, s));
     int *counts = static_cast<int *>(scratch);
     int *offsets = counts + nt;  // nt + 1 entries
+    int *bsum = offsets + nt + 1;  // nb entries
     float4 *binned = reinterpret_cast<float4 *>(
-        (reinterpret_cast<uintptr_t>(offsets + nt + 1) + 15) & ~uintptr_t(15));
+        (reinterpret_cast<uintptr_t>(bsum + nb) + 15) & ~uintptr_t(15));
     const unsigned fb = (unsigned)std::min<long>((nt + 255) / 256, 16384);
     fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
     TDM_LAUNCHED();
@@ -1018,7 +1074,9 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
             tx, ty, counts);
         TDM_LAUNCHED();
     }
-    acc_scan_kernel<<<1, 1024, 0, s>>>(counts, nt, offsets);
+    acc_blocksum_kernel<<<(unsigned)nb, 1024, 0, s>>>(counts, nt, bsum);
+    TDM_LAUNCHED();
+    acc_offsets_kernel<<<(unsigned)nb, 1024, 0, s>>>(counts, nt, bsum, offsets);
     TDM_LAUNCHED();
     fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
     TDM_LAUNCHED();
