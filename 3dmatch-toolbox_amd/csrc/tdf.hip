@@ -635,14 +635,72 @@ __device__ inline long acc_tile_of(int gx, int gy, int gz, int tx, int ty) {
     return ((long)(gz / kAT) * ty + gy / kAT) * tx + gx / kAT;
 }
 
-__global__ void acc_count_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
-                                 float ox, float oy, float oz, float vs, int tx, int ty,
-                                 int *__restrict__ counts) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int gx, gy, gz;
-    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
-    atomicAdd(&counts[acc_tile_of(gx, gy, gz, tx, ty)], 1);
+// Binning of the points by 8x8x8 tile with the bins of a workgroup's 4096
+// points aggregated in an LDS hash table (tile -> count), so the device-scope
+// atomics on the clustered bins come once per distinct bin and workgroup.
+constexpr int kBinThreads = 1024, kBinPts = 4, kBinSlots = 8192;  // slots: twice the keys
+
+struct BinTable {
+    unsigned key[kBinSlots];
+    unsigned cnt[kBinSlots];  // count, then (fill) the slot's first index in binned
+};
+
+// Decodes this thread's kBinPts points (tile index or kScatEmpty) and counts
+// them into the table; slot and rank per point.  Every thread calls it.
+__device__ inline void bin_points(BinTable &T, const float *__restrict__ pts, int n, int dx,
+                                  int dy, int dz, float ox, float oy, float oz, float vs, int tx,
+                                  int ty, float4 (&p)[kBinPts], int (&slot)[kBinPts],
+                                  unsigned (&rank)[kBinPts]) {
+    for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
+        T.key[i] = kScatEmpty;
+        T.cnt[i] = 0u;
+    }
+    const long base = (long)blockIdx.x * kBinThreads * kBinPts + threadIdx.x;
+    unsigned key[kBinPts];
+#pragma unroll
+    for (int k = 0; k < kBinPts; ++k) {
+        const long i = min(base + (long)k * kBinThreads, (long)n - 1);
+        p[k] = make_float4(pts[i * 3 + 0], pts[i * 3 + 1], pts[i * 3 + 2], 0.0f);
+    }
+#pragma unroll
+    for (int k = 0; k < kBinPts; ++k) {
+        // point_voxel's arithmetic on the loaded coordinates
+        const float fx = roundf((p[k].x - ox) / vs);
+        const float fy = roundf((p[k].y - oy) / vs);
+        const float fz = roundf((p[k].z - oz) / vs);
+        const bool ok = base + (long)k * kBinThreads < n && fx >= 0.0f && fx < (float)dx &&
+                        fy >= 0.0f && fy < (float)dy && fz >= 0.0f && fz < (float)dz;
+        key[k] = ok ? (unsigned)acc_tile_of((int)fx, (int)fy, (int)fz, tx, ty) : kScatEmpty;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBinPts; ++k) {
+        slot[k] = -1;
+        if (key[k] == kScatEmpty) continue;
+        unsigned h = (key[k] * 2654435761u) & (kBinSlots - 1);
+        for (int probe = 0; probe < kBinSlots; ++probe) {
+            const unsigned prev = atomicCAS(&T.key[h], kScatEmpty, key[k]);
+            if (prev == kScatEmpty || prev == key[k]) {
+                rank[k] = atomicAdd(&T.cnt[h], 1u);
+                slot[k] = (int)h;
+                break;
+            }
+            h = (h + 1) & (kBinSlots - 1);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBinThreads) void acc_count_kernel(
+    const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
+    float vs, int tx, int ty, int *__restrict__ counts) {
+    __shared__ BinTable T;
+    float4 p[kBinPts];
+    int slot[kBinPts];
+    unsigned rank[kBinPts];
+    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, tx, ty, p, slot, rank);
+    for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads)
+        if (T.key[i] != kScatEmpty) atomicAdd(&counts[T.key[i]], (int)T.cnt[i]);
 }
 
 // Exclusive scan of n counts in two launches: per-1024 block sums, then
@@ -692,17 +750,26 @@ __global__ __launch_bounds__(1024) void acc_offsets_kernel(const int *__restrict
     if (i == n - 1) offsets[n] = excl + v;
 }
 
-__global__ void acc_fill_kernel(const float *__restrict__ pts, int n, int dx, int dy, int dz,
-                                float ox, float oy, float oz, float vs, int tx, int ty,
-                                const int *__restrict__ offsets, int *__restrict__ cursor,
-                                float4 *__restrict__ binned) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int gx, gy, gz;
-    if (!point_voxel(pts, i, dx, dy, dz, ox, oy, oz, vs, gx, gy, gz)) return;
-    const long t = acc_tile_of(gx, gy, gz, tx, ty);
-    const int slot = atomicAdd(&cursor[t], 1);
-    binned[offsets[t] + slot] = make_float4(pts[3L * i], pts[3L * i + 1], pts[3L * i + 2], 0.0f);
+// Bin i's points go to binned[offsets[i] ..): a workgroup reserves the range
+// of each of its bins with one atomic on cursor[] and places its points by
+// their rank in the LDS count (order within a bin does not matter: the
+// kernel below takes a minimum).
+__global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
+    const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
+    float vs, int tx, int ty, const int *__restrict__ offsets, int *__restrict__ cursor,
+    float4 *__restrict__ binned) {
+    __shared__ BinTable T;
+    float4 p[kBinPts];
+    int slot[kBinPts];
+    unsigned rank[kBinPts];
+    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, tx, ty, p, slot, rank);
+    for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads)
+        if (T.key[i] != kScatEmpty)
+            T.cnt[i] = (unsigned)offsets[T.key[i]] + (unsigned)atomicAdd(&cursor[T.key[i]], (int)T.cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBinPts; ++k)
+        if (slot[k] >= 0) binned[T.cnt[slot[k]] + rank[k]] = p[k];
 }
 
 // One 8x8x8 tile per workgroup of 512 threads.  Points of the tiles within
@@ -1069,7 +1136,8 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
     TDM_LAUNCHED();
     if (num_pts > 0) {
-        acc_count_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+        acc_count_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
+                           kBinThreads, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
             tx, ty, counts);
         TDM_LAUNCHED();
@@ -1081,7 +1149,8 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
     TDM_LAUNCHED();
     if (num_pts > 0) {
-        acc_fill_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+        acc_fill_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
+                          kBinThreads, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
             tx, ty, offsets, counts, binned);
         TDM_LAUNCHED();
