@@ -617,7 +617,31 @@ struct Conv1Args {
     int N;
     int relu;
     float oscale;  // fp8 outputs: multiplier before conversion
+    // n / (P - 2) = umulhi(n, od_mul) >> od_sh for n < 2^31 (od_mul = 0:
+    // P - 2 == 1); used when M < 2^31, else the row decode divides in 64 bits
+    unsigned od_mul;
+    int od_sh;
 };
+
+// Multiply-high division by d for n < 2^31 (Granlund and Montgomery's
+// round-up method: l = ceil(log2 d), m = ceil(2^(31 + l) / d) < 2^32, then
+// n / d = (n * m) >> (31 + l); exact because m * d - 2^(31 + l) < d <= 2^l).
+void fast_div(int d, unsigned *mul, int *sh) {
+    if (d == 1) {
+        *mul = 0u;
+        *sh = 0;
+        return;
+    }
+    int l = 0;
+    while ((1L << l) < d) ++l;
+    const unsigned long long num = 1ull << (31 + l);
+    *mul = (unsigned)((num + (unsigned long long)d - 1) / (unsigned long long)d);
+    *sh = l - 1;
+}
+
+__device__ inline unsigned div_od(unsigned n, const Conv1Args &a) {
+    return a.od_mul ? __umulhi(n, a.od_mul) >> a.od_sh : n;
+}
 
 // MODE 0: patches [batch][P^3]; 1: keypoints in one grid; 2: keypoints in
 // a table of grids (gid[b] picks patch b's grid).
@@ -642,15 +666,27 @@ __global__ __launch_bounds__(256) void conv1_kernel(Conv1Args a) {
     bool inb[4];
     int ex[4], ey[4], ez[4], sy[4];
     long sz[4];
+    const bool small = a.M < (1L << 31);  // rows fit 31 bits: multiply-high division
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         long m = min(m0 + 16 * i + r16, a.M - 1);
-        const int px = (int)(m % OD);
-        long q = m / OD;
-        const int py = (int)(q % OD);
-        q /= OD;
-        const int pz = (int)(q % OD);
-        const long b = q / OD;
+        int px, py, pz;
+        long b;
+        if (small) {
+            const unsigned u = (unsigned)m;
+            const unsigned q1 = div_od(u, a), q2 = div_od(q1, a), q3 = div_od(q2, a);
+            px = (int)(u - q1 * (unsigned)OD);
+            py = (int)(q1 - q2 * (unsigned)OD);
+            pz = (int)(q2 - q3 * (unsigned)OD);
+            b = q3;
+        } else {
+            px = (int)(m % OD);
+            long q = m / OD;
+            py = (int)(q % OD);
+            q /= OD;
+            pz = (int)(q % OD);
+            b = q / OD;
+        }
         if (GRID) {
             const float *base = a.base;
             ex[i] = a.dx;
@@ -1033,6 +1069,7 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
     a.oscale = oscale;
     const long OD = src.P - 2;
     a.M = (long)batch * OD * OD * OD;
+    fast_div((int)OD, &a.od_mul, &a.od_sh);
     dim3 grid((unsigned)((a.M + 255) / 256), N / 64);
     a.grids = src.grids;
     a.gid = src.gid;
