@@ -464,6 +464,7 @@ struct CloudDev {
     long pt_off;    // first point of this cloud in the batch's point numbering
     long word_off;  // first bit word of this cloud's grid in the batch bit buffer
     long tile_off;  // first tile of this cloud in the batch's tile numbering
+    long vox_off;   // first voxel of this cloud in the batch's voxel numbering
     int n;
     int dims[3];
     float origin[3];
@@ -483,7 +484,8 @@ __device__ inline int find_cloud(const CloudDev *__restrict__ clouds, int num, l
 }
 
 __global__ void scatter_bits_batch_kernel(const CloudDev *__restrict__ clouds, int num,
-                                          long total, float vs, unsigned *__restrict__ bits) {
+                                          long total, float vs, unsigned *__restrict__ bits,
+                                          unsigned char *__restrict__ flags) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (long)gridDim.x * blockDim.x) {
         const int c = find_cloud(clouds, num, i, [](const CloudDev &d) { return d.pt_off; });
@@ -497,12 +499,89 @@ __global__ void scatter_bits_batch_kernel(const CloudDev *__restrict__ clouds, i
         const long w =
             cd.word_off + ((long)gz * cd.dims[1] + gy) * row_words(cd.dims[0]) + (gx >> 5);
         atomicOr(&bits[w], 1u << (gx & 31));
+        flags[cd.tile_off + tile_of(gx, gy, gz, cd.dims[0], cd.dims[1])] = 1;
+    }
+}
+
+// The batch scatter with the words of a workgroup's 4096 points deduplicated
+// in LDS before the device-scope atomics (as scatter_bits_dedup_kernel);
+// keys are batch word indices, below 0xffffffff.
+__global__ __launch_bounds__(kScatThreads) void scatter_bits_batch_dedup_kernel(
+    const CloudDev *__restrict__ clouds, int num, long total, float vs,
+    unsigned *__restrict__ bits, unsigned char *__restrict__ flags) {
+    __shared__ unsigned keys[kScatSlots];
+    __shared__ unsigned vals[kScatSlots];
+    for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads) {
+        keys[s] = kScatEmpty;
+        vals[s] = 0u;
+    }
+    const long base = (long)blockIdx.x * kScatThreads * kScatPts + threadIdx.x;
+    // the workgroup's points are consecutive: one search for the first, then
+    // each point steps forward over the (few) cloud boundaries before it
+    const int c0 = find_cloud(clouds, num, (long)blockIdx.x * kScatThreads * kScatPts,
+                              [](const CloudDev &d) { return d.pt_off; });
+    unsigned w[kScatPts], bit[kScatPts];
+#pragma unroll
+    for (int k = 0; k < kScatPts; ++k) {
+        w[k] = kScatEmpty;
+        bit[k] = 0u;
+        const long i = base + (long)k * kScatThreads;
+        if (i >= total) continue;
+        int c = c0;
+        while (c + 1 < num && clouds[c + 1].pt_off <= i) ++c;
+        const CloudDev &cd = clouds[c];
+        const long j = i - cd.pt_off;
+        if (j >= cd.n) continue;  // (an empty cloud shares its offset with the next)
+        int gx, gy, gz;
+        if (!point_voxel(cd.pts, j, cd.dims[0], cd.dims[1], cd.dims[2], cd.origin[0],
+                         cd.origin[1], cd.origin[2], vs, gx, gy, gz))
+            continue;
+        w[k] = (unsigned)(cd.word_off + ((long)gz * cd.dims[1] + gy) * row_words(cd.dims[0]) +
+                          (gx >> 5));
+        bit[k] = 1u << (gx & 31);
+        flags[cd.tile_off + tile_of(gx, gy, gz, cd.dims[0], cd.dims[1])] = 1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScatPts; ++k) {
+        if (w[k] == kScatEmpty) continue;
+        unsigned h = (w[k] * 2654435761u) & (kScatSlots - 1);
+        for (int probe = 0; probe < kScatSlots; ++probe) {
+            const unsigned prev = atomicCAS(&keys[h], kScatEmpty, w[k]);
+            if (prev == kScatEmpty || prev == w[k]) {
+                atomicOr(&vals[h], bit[k]);
+                break;
+            }
+            h = (h + 1) & (kScatSlots - 1);
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kScatSlots; s += kScatThreads)
+        if (keys[s] != kScatEmpty) atomicOr(&bits[keys[s]], vals[s]);
+}
+
+// Zero every cloud's output grid: workgroup b takes voxels [b*chunk,
+// (b+1)*chunk) of the batch's voxel numbering, cloud by cloud (a chunk may
+// span clouds; the cloud of its start is found once).
+__global__ void zero_clouds_kernel(const CloudDev *__restrict__ clouds, int num, long total_vox,
+                                   long chunk) {
+    long v = (long)blockIdx.x * chunk;
+    const long end = min(v + chunk, total_vox);
+    int c = find_cloud(clouds, num, v, [](const CloudDev &d) { return d.vox_off; });
+    while (v < end) {
+        const CloudDev &cd = clouds[c];
+        const long cend = cd.vox_off + (long)cd.dims[0] * cd.dims[1] * cd.dims[2];
+        const long stop = min(end, cend);
+        for (long e = v + threadIdx.x; e < stop; e += blockDim.x) cd.tdf[e - cd.vox_off] = 0.0f;
+        v = stop;
+        ++c;
     }
 }
 
 template <int R>
 __global__ __launch_bounds__(kTdfThreads) void tdf_tile_batch_kernel(
-    const CloudDev *__restrict__ clouds, int num, const unsigned *__restrict__ bits, Lut lut) {
+    const CloudDev *__restrict__ clouds, int num, const unsigned *__restrict__ bits, Lut lut,
+    const unsigned char *__restrict__ flags) {
     __shared__ TileLds<R> L;
     const long t = blockIdx.x;
     const int c = find_cloud(clouds, num, t, [](const CloudDev &d) { return d.tile_off; });
@@ -512,7 +591,7 @@ __global__ __launch_bounds__(kTdfThreads) void tdf_tile_batch_kernel(
     const int ty = (int)((lt / cd.tiles[0]) % cd.tiles[1]);
     const int tz = (int)(lt / ((long)cd.tiles[0] * cd.tiles[1]));
     tdf_tile<R, kTdfThreads>(L, bits + cd.word_off, cd.dims[0], cd.dims[1], cd.dims[2], lut, cd.tdf, tx * kTX,
-                ty * kTY, tz * kTZ, nullptr);
+                ty * kTY, tz * kTZ, flags + cd.tile_off);
 }
 
 // float32 decode of core/demo.cu:32-34.
@@ -1172,7 +1251,9 @@ struct tdm_tdf_plan {
     long total_pts = 0, total_tiles = 0;
     size_t total_words = 0;
     CloudDev *table = nullptr;   // device
-    unsigned *bits = nullptr;    // device, all clouds' bit grids
+    unsigned *bits = nullptr;    // device, all clouds' bit grids, then the tile flags
+    unsigned char *flags = nullptr;
+    long total_vox = 0;
     std::vector<CloudDev> host;  // the same table on the host
 };
 
@@ -1192,7 +1273,7 @@ int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, floa
     plan->decode = decode;
     plan->vs = voxel_size;
     plan->host.resize(num);
-    long pts = 0, tiles = 0;
+    long pts = 0, tiles = 0, vox = 0;
     size_t words = 0;
     for (int c = 0; c < num; ++c) {
         const tdm_cloud_ref &in = clouds[c];
@@ -1215,6 +1296,8 @@ int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, floa
         d.pt_off = pts;
         d.word_off = (long)words;
         d.tile_off = tiles;
+        d.vox_off = vox;
+        vox += (long)in.dim_x * in.dim_y * in.dim_z;
         pts += in.num_pts;
         tiles += (long)d.tiles[0] * d.tiles[1] * d.tiles[2];
         words += (size_t)row_words(in.dim_x) * in.dim_y * in.dim_z;
@@ -1223,8 +1306,11 @@ int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, floa
     plan->total_pts = pts;
     plan->total_tiles = tiles;
     plan->total_words = words;
+    plan->total_vox = vox;
     TDM_HIP(hipMalloc((void **)&plan->table, sizeof(CloudDev) * num));
-    TDM_HIP(hipMalloc((void **)&plan->bits, words * sizeof(unsigned)));
+    // the tile flags (one byte per tile) follow the bits, padded to a word
+    TDM_HIP(hipMalloc((void **)&plan->bits, (words + ((size_t)tiles + 3) / 4) * sizeof(unsigned)));
+    plan->flags = reinterpret_cast<unsigned char *>(plan->bits + words);
     TDM_HIP(hipMemcpy(plan->table, plan->host.data(), sizeof(CloudDev) * num,
                       hipMemcpyHostToDevice));
     *out = plan.release();
@@ -1236,20 +1322,34 @@ int tdm_tdf_plan_run(tdm_tdf_plan *plan, void *stream) {
     TDM_CHECK(plan, TDM_ERR_INVALID_ARG, "NULL plan");
     TDM_HIP(hipSetDevice(plan->device));
     hipStream_t s = tdm::as_stream(stream);
-    const unsigned fb = (unsigned)std::min<size_t>(tdm::blocks_for(plan->total_words, 256), 16384);
-    fill_u32_kernel<<<fb, 256, 0, s>>>(plan->bits, plan->total_words, 0u);
+    // the grids are zeroed first; the tile kernel writes only tiles whose halo
+    // holds a point
+    const long chunk = 65536;
+    zero_clouds_kernel<<<(unsigned)((plan->total_vox + chunk - 1) / chunk), 256, 0, s>>>(
+        plan->table, plan->num, plan->total_vox, chunk);
     TDM_LAUNCHED();
-    if (plan->total_pts > 0) {
+    const size_t cw = plan->total_words + ((size_t)plan->total_tiles + 3) / 4;
+    const unsigned fb = (unsigned)std::min<size_t>(tdm::blocks_for(cw, 256), 16384);
+    fill_u32_kernel<<<fb, 256, 0, s>>>(plan->bits, cw, 0u);
+    TDM_LAUNCHED();
+    if (plan->total_pts > 0 && plan->total_words < (size_t)kScatEmpty) {
+        const long per_block = (long)kScatThreads * kScatPts;
+        scatter_bits_batch_dedup_kernel<<<(unsigned)((plan->total_pts + per_block - 1) / per_block),
+                                          kScatThreads, 0, s>>>(plan->table, plan->num,
+                                                                plan->total_pts, plan->vs,
+                                                                plan->bits, plan->flags);
+        TDM_LAUNCHED();
+    } else if (plan->total_pts > 0) {
         const unsigned sb = (unsigned)std::min<long>((plan->total_pts + 255) / 256, 65536);
         scatter_bits_batch_kernel<<<sb, 256, 0, s>>>(plan->table, plan->num, plan->total_pts,
-                                                      plan->vs, plan->bits);
+                                                      plan->vs, plan->bits, plan->flags);
         TDM_LAUNCHED();
     }
     const Lut lut = make_lut(plan->r);
     const unsigned tb = (unsigned)plan->total_tiles;
     switch (plan->r) {
 #define TDM_TB(R) \
-    case R: tdf_tile_batch_kernel<R><<<tb, kTdfThreads, 0, s>>>(plan->table, plan->num, plan->bits, lut); break;
+    case R: tdf_tile_batch_kernel<R><<<tb, kTdfThreads, 0, s>>>(plan->table, plan->num, plan->bits, lut, plan->flags); break;
         TDM_TB(1) TDM_TB(2) TDM_TB(3) TDM_TB(4) TDM_TB(5) TDM_TB(6) TDM_TB(7) TDM_TB(8)
 #undef TDM_TB
     default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", plan->r);
