@@ -1307,12 +1307,21 @@ int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, floa
     plan->total_tiles = tiles;
     plan->total_words = words;
     plan->total_vox = vox;
-    TDM_HIP(hipMalloc((void **)&plan->table, sizeof(CloudDev) * num));
     // the tile flags (one byte per tile) follow the bits, padded to a word
-    TDM_HIP(hipMalloc((void **)&plan->bits, (words + ((size_t)tiles + 3) / 4) * sizeof(unsigned)));
+    const size_t bit_bytes = (words + ((size_t)tiles + 3) / 4) * sizeof(unsigned);
+    hipError_t e = hipMalloc((void **)&plan->table, sizeof(CloudDev) * num);
+    if (e == hipSuccess) e = hipMalloc((void **)&plan->bits, bit_bytes);
+    if (e == hipSuccess)
+        e = hipMemcpy(plan->table, plan->host.data(), sizeof(CloudDev) * num,
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (plan->table) (void)hipFree(plan->table);
+        if (plan->bits) (void)hipFree(plan->bits);
+        return tdm::fail(e == hipErrorOutOfMemory ? TDM_ERR_NOMEM : TDM_ERR_HIP,
+                         "TDF plan of %zu bytes: %s", bit_bytes, hipGetErrorString(e));
+    }
     plan->flags = reinterpret_cast<unsigned char *>(plan->bits + words);
-    TDM_HIP(hipMemcpy(plan->table, plan->host.data(), sizeof(CloudDev) * num,
-                      hipMemcpyHostToDevice));
     *out = plan.release();
     return TDM_OK;
 }
