@@ -645,11 +645,12 @@ __device__ inline unsigned div_od(unsigned n, const Conv1Args &a) {
 
 // MODE 0: patches [batch][P^3]; 1: keypoints in one grid; 2: keypoints in
 // a table of grids (gid[b] picks patch b's grid).
-// Four workgroups per CU for 16- and 8-bit outputs (as many as their LDS
-// staging allows; the register budget that follows keeps the gathers' latency
-// covered: 1.42 -> 1.19 ms per 1024-patch fp16 pass), two for float outputs.
+// Four workgroups per CU (as many as the LDS staging of the output allows;
+// the register budget that follows keeps the gathers' latency covered:
+// 1.42 -> 1.19 ms per 1024-patch fp16 pass); three with fp32 operands, whose
+// registers do not fit four without spilling.
 template <typename T, typename OT, int MODE>
-__global__ __launch_bounds__(256, sizeof(OT) < 4 ? 4 : 2) void conv1_kernel(Conv1Args a) {
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv1Args a) {
     constexpr bool GRID = MODE != 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long m0 = ((long)blockIdx.x * 4 + wave) * 64;
@@ -779,35 +780,43 @@ __global__ __launch_bounds__(256, sizeof(OT) < 4 ? 4 : 2) void conv1_kernel(Conv
     // of 4 rows per lane; staging the wave's 64x64 tile lets every lane write
     // whole 16-byte pieces of contiguous output rows (conv1 is bound by its
     // output writes: 28^3 x 64 values per patch).
-    constexpr int PITCH = 64 + 16 / (int)sizeof(OT);  // padded row, in OT elements
+    // Float outputs go in two 32-column halves, so every output type stages
+    // 36 KB per workgroup (four workgroups per CU).
+    constexpr int HALVES = sizeof(OT) == 4 ? 2 : 1;
+    constexpr int COLS = 64 / HALVES;
+    constexpr int PITCH = COLS + 16 / (int)sizeof(OT);  // padded row, in OT elements
     __shared__ __attribute__((aligned(16))) OT tile[4][64 * PITCH];
     OT *__restrict__ t = tile[wave];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int cl = 16 * j + r16;
-        const float bv = a.bias[n0 + cl];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float v = acc[i][j][r] + bv;
-                if (a.relu) v = v > 0.0f ? v : 0.0f;
-                if (sizeof(OT) == 1) v *= a.oscale;
-                store<OT>(t + (16 * i + 4 * g + r) * PITCH + cl, v);
-            }
-    }
-    __syncthreads();
     OT *__restrict__ out = static_cast<OT *>(a.out);
-    constexpr int PIECES = 64 * (int)sizeof(OT) / 16;  // 16-byte pieces per row
-    constexpr int PER = 16 / (int)sizeof(OT);          // elements per piece
+    constexpr int PIECES = COLS * (int)sizeof(OT) / 16;  // 16-byte pieces per half row
+    constexpr int PER = 16 / (int)sizeof(OT);            // elements per piece
 #pragma unroll
-    for (int it = 0; it < 64 * PIECES / 64; ++it) {
-        const int idx = it * 64 + lane;
-        const int rl = idx / PIECES, pc = idx % PIECES;
-        const long row = m0 + rl;
-        if (row < a.M)
-            *reinterpret_cast<uint4 *>(out + row * a.N + n0 + pc * PER) =
-                *reinterpret_cast<const uint4 *>(t + rl * PITCH + pc * PER);
+    for (int h = 0; h < HALVES; ++h) {
+#pragma unroll
+        for (int j = h * (4 / HALVES); j < (h + 1) * (4 / HALVES); ++j) {
+            const int cl = 16 * j + r16;
+            const float bv = a.bias[n0 + cl];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = acc[i][j][r] + bv;
+                    if (a.relu) v = v > 0.0f ? v : 0.0f;
+                    if (sizeof(OT) == 1) v *= a.oscale;
+                    store<OT>(t + (16 * i + 4 * g + r) * PITCH + cl - h * COLS, v);
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < 64 * PIECES / 64; ++it) {
+            const int idx = it * 64 + lane;
+            const int rl = idx / PIECES, pc = idx % PIECES;
+            const long row = m0 + rl;
+            if (row < a.M)
+                *reinterpret_cast<uint4 *>(out + row * a.N + n0 + h * COLS + pc * PER) =
+                    *reinterpret_cast<const uint4 *>(t + rl * PITCH + pc * PER);
+        }
+        if (h + 1 < HALVES) __syncthreads();  // the half is copied out before the next overwrites it
     }
 }
 
