@@ -314,21 +314,21 @@ struct TileLds {
 template <int R>
 __device__ inline unsigned char xpass(unsigned w0, unsigned w1, unsigned w2, unsigned w3, int lane) {
     if (!(w0 | w1 | w2 | w3)) return kNone;
-    const unsigned long long lo = ((unsigned long long)w1 << 32) | w0;
-    const unsigned long long hi = ((unsigned long long)w3 << 32) | w2;
-    // Window of 2R+1 bits starting at local bit lane + 32 - R.
-    const int s = lane + 32 - R;
-    unsigned long long win;
-    if (s == 0) win = lo;
-    else if (s < 64) win = (lo >> s) | (hi << (64 - s));
-    else win = hi >> (s - 64);
-    win &= (1ull << (2 * R + 1)) - 1;
+    // The window of 2R+1 (<= 17) bits starting at bit s = lane + 32 - R of the
+    // 128-bit row w3:w2:w1:w0 lies in words k = s / 32 and k + 1 (s % 32 + 2R
+    // < 64), so one 32-bit funnel shift extracts it.
+    const int s = lane + 32 - R;  // in [32 - R, 95 - R]: k is 0, 1 or 2
+    const int k = s >> 5;
+    const unsigned lo = k == 0 ? w0 : k == 1 ? w1 : w2;
+    const unsigned hi = k == 0 ? w1 : k == 1 ? w2 : w3;
+    unsigned win = __builtin_amdgcn_alignbit(hi, lo, (unsigned)(s & 31));
+    win &= (1u << (2 * R + 1)) - 1u;
     if (!win) return kNone;
-    const unsigned long long right = win >> R;              // bits at >= x
-    const unsigned long long left = win & ((2ull << R) - 1);  // bits at <= x
+    const unsigned right = win >> R;               // bits at >= x
+    const unsigned left = win & ((2u << R) - 1u);  // bits at <= x
     int d = 255;
-    if (right) d = __builtin_ctzll(right);
-    if (left) d = min(d, R - (63 - __builtin_clzll(left)));
+    if (right) d = __builtin_ctz(right);
+    if (left) d = min(d, R - (31 - __builtin_clz(left)));
     return (unsigned char)(d * d);
 }
 
