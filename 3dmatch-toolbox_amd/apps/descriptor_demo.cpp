@@ -3,7 +3,8 @@
 //
 //   descriptor_demo <cloud.ply> <out_prefix> [--num-keypoints N] [--seed S]
 //       [--keypoint-indices FILE] [--net JSON] [--weights MARVIN]
-//       [--precision fp32|bf16] [--decode exact|reference] [--tdf-bin PATH]
+//       [--precision fp32|fp16|bf16|fp8] [--decode exact|reference]
+//       [--tdf-bin PATH]
 //       [--batch B]
 //
 // Writes <out_prefix>.keypts.bin and <out_prefix>.desc.3dmatch.bin in the
@@ -15,6 +16,7 @@
 // 3dmatch-weights-snapshot-137000.marvin (both in the working directory).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -22,24 +24,8 @@
 #include <string>
 #include <vector>
 
+#include "cli_common.hpp"
 #include "tdm.h"
-
-#define CHECK(call)                                                                        \
-    do {                                                                                   \
-        int s_ = (call);                                                                   \
-        if (s_ != TDM_OK) {                                                                \
-            std::fprintf(stderr, "error: %s: %s\n", tdm_status_string(s_), tdm_last_error()); \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
-#define HIPCHECK(call)                                                                     \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            std::fprintf(stderr, "HIP error: %s\n", hipGetErrorString(e_));                \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
 
 int main(int argc, char **argv) {
     if (argc < 3) {
@@ -65,9 +51,10 @@ int main(int argc, char **argv) {
         else if (a == "--batch") batch = std::atoi(next().c_str());
         else if (a == "--precision") {
             std::string p = next();
-            if (p == "fp32") precision = TDM_PREC_FP32;
-            else if (p == "bf16") precision = TDM_PREC_BF16;
-            else { std::fprintf(stderr, "unknown precision %s\n", p.c_str()); return 1; }
+            if (!parse_precision(p, &precision)) {
+                std::fprintf(stderr, "unknown precision %s (fp32|fp16|bf16|fp8)\n", p.c_str());
+                return 1;
+            }
         } else if (a == "--decode") {
             std::string d = next();
             if (d == "exact") decode = TDM_DECODE_EXACT;
@@ -137,6 +124,15 @@ int main(int argc, char **argv) {
     HIPCHECK(hipMalloc(&d_kp, sizeof(int) * kp_grid.size() + 4));
     HIPCHECK(hipMalloc(&d_desc, sizeof(float) * (size_t)num_keypts * desc_size + 4));
     HIPCHECK(hipMemcpy(d_kp, kp_grid.data(), sizeof(int) * kp_grid.size(), hipMemcpyHostToDevice));
+    if (precision == TDM_PREC_FP8 && num_keypts > 0) {
+        const int nc = std::min(num_keypts, kCalibrationPatches), P = net_patch_extent(net);
+        float *d_cal = nullptr;
+        HIPCHECK(hipMalloc(&d_cal, sizeof(float) * (size_t)nc * P * P * P));
+        CHECK(tdm_extract_patches(nullptr, d_tdf, dims[0], dims[1], dims[2], d_kp, nc, P, d_cal,
+                                  nullptr));
+        CHECK(calibrate_fp8(net, net_json, weights, d_cal, nc));
+        (void)hipFree(d_cal);
+    }
     std::printf("Computing 3DMatch descriptors for %d keypoints...\n", num_keypts);
     CHECK(tdm_net_forward_grid(net, d_tdf, dims[0], dims[1], dims[2], d_kp, num_keypts, d_desc, nullptr));
     std::vector<float> desc((size_t)num_keypts * desc_size);

@@ -3,42 +3,51 @@
 // tsdf.ply (surface voxels |tsdf| < 0.2, weight > 1).
 //
 //   fusion_demo [camK dataPath baseIdx firstIdx nFrames ox oy oz voxelSize truncMargin]
-//               [--dims X Y Z] [--tsdf-bin PATH] [--weight-bin PATH] [--fused] [--out PLY]
+//               [--dims X Y Z] [--tsdf-bin PATH] [--weight-bin PATH] [--out PLY]
+//               [--serial | --fused] [--threads N]
 //
 // Positional arguments and defaults are the reference's (500^3 grid at 6 mm,
-// origin (-1.5, -1.5, 0.5), truncation 5 voxels).  --fused uploads all frames
-// and integrates them in one pass (bit-identical result).
+// origin (-1.5, -1.5, 0.5), truncation 5 voxels).  By default the frames go
+// through the overlapped pipeline (tdm_fusion_*: PNG decoder threads, pinned
+// ring, u16 upload, conversion and Integrate on the GPU in frame order).
+// --serial runs the reference's loop shape (decode, copy, integrate, one
+// frame at a time); --fused uploads all frames and integrates them in one
+// pass.  All three give bit-identical grids.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "cli_common.hpp"
 #include "tdm.h"
-
-#define CHECK(call)                                                                        \
-    do {                                                                                   \
-        int s_ = (call);                                                                   \
-        if (s_ != TDM_OK) {                                                                \
-            std::fprintf(stderr, "error: %s: %s\n", tdm_status_string(s_), tdm_last_error()); \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
-#define HIPCHECK(call)                                                                     \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            std::fprintf(stderr, "HIP error: %s\n", hipGetErrorString(e_));                \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
 
 static std::string frame_prefix(const std::string &dir, int idx) {
     char buf[32];
     std::snprintf(buf, sizeof buf, "/frame-%06d", idx);
     return dir + buf;
+}
+
+// tsdf.ply (depth-fusion/demo.cu:186-194) and the optional grid files.
+static int write_outputs(const std::string &out_ply, const std::string &tsdf_bin,
+                         const std::string &weight_bin, const int dims[3], const float origin[3],
+                         float voxel_size, float trunc_margin, const std::vector<float> &tsdf,
+                         const std::vector<float> &weight) {
+    std::printf("Saving surface point cloud (%s)...\n", out_ply.c_str());
+    int npts = 0;
+    CHECK(tdm_save_surface_ply(out_ply.c_str(), dims, voxel_size, origin, tsdf.data(), weight.data(),
+                               0.2f, 1.0f, &npts));
+    if (!tsdf_bin.empty()) {
+        std::printf("Saving TSDF voxel grid values to disk (%s)...\n", tsdf_bin.c_str());
+        CHECK(tdm_write_grid_bin(tsdf_bin.c_str(), dims, origin, voxel_size, trunc_margin, tsdf.data()));
+    }
+    if (!weight_bin.empty())
+        CHECK(tdm_write_grid_bin(weight_bin.c_str(), dims, origin, voxel_size, trunc_margin,
+                                 weight.data()));
+    return 0;
 }
 
 int main(int argc, char **argv) {
@@ -52,7 +61,8 @@ int main(int argc, char **argv) {
     float trunc_margin = voxel_size * 5;
     int dims[3] = {500, 500, 500};
     std::string tsdf_bin, weight_bin, out_ply = "tsdf.ply";
-    bool fused = false;
+    bool fused = false, serial = false;
+    int threads = 0;
 
     std::vector<std::string> pos;
     for (int i = 1; i < argc; ++i) {
@@ -67,6 +77,13 @@ int main(int argc, char **argv) {
             out_ply = argv[++i];
         } else if (a == "--fused") {
             fused = true;
+        } else if (a == "--serial") {
+            serial = true;
+        } else if (a == "--threads" && i + 1 < argc) {
+            threads = std::atoi(argv[++i]);
+        } else if (a.size() > 2 && a.compare(0, 2, "--") == 0) {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 1;
         } else {
             pos.push_back(a);
         }
@@ -93,7 +110,37 @@ int main(int argc, char **argv) {
     CHECK(tdm_load_matrix((frame_prefix(data_path, base_frame_idx) + ".pose.txt").c_str(), 4, 4, base2world));
     CHECK(tdm_invert_matrix(base2world, base2world_inv));
 
+    if (fused && serial) {
+        std::fprintf(stderr, "--fused and --serial exclude each other\n");
+        return 1;
+    }
     const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    const int nf = (int)num_frames;
+    const float origin[3] = {ox, oy, oz};
+    std::vector<float> tsdf(nvox), weight(nvox);
+    if (!fused && !serial) {  // the frame loop of depth-fusion/demo.cu:156-183, overlapped
+        tdm_fusion *fu = nullptr;
+        CHECK(tdm_fusion_create(nullptr, cam_K, H, W, dims, origin, voxel_size, trunc_margin, threads,
+                                &fu));
+        for (int f = 0; f < nf; ++f) {
+            const std::string pre = frame_prefix(data_path, first_frame_idx + f);
+            const std::string depth_file = pre + ".depth.png";
+            float cam2world[16], cam2base[16];
+            CHECK(tdm_load_matrix((pre + ".pose.txt").c_str(), 4, 4, cam2world));
+            CHECK(tdm_multiply_matrix(base2world_inv, cam2world, cam2base));
+            std::printf("Fusing: %s\n", depth_file.c_str());
+            CHECK(tdm_fusion_add_png(fu, depth_file.c_str(), cam2base));
+        }
+        CHECK(tdm_fusion_wait(fu));
+        float *g_tsdf = nullptr, *g_weight = nullptr;
+        CHECK(tdm_fusion_grids(fu, &g_tsdf, &g_weight));
+        HIPCHECK(hipMemcpy(tsdf.data(), g_tsdf, nvox * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(weight.data(), g_weight, nvox * sizeof(float), hipMemcpyDeviceToHost));
+        CHECK(tdm_fusion_destroy(fu));
+        return write_outputs(out_ply, tsdf_bin, weight_bin, dims, origin, voxel_size, trunc_margin,
+                             tsdf, weight);
+    }
+
     float *d_tsdf = nullptr, *d_weight = nullptr, *d_K = nullptr;
     HIPCHECK(hipMalloc(&d_tsdf, nvox * sizeof(float)));
     HIPCHECK(hipMalloc(&d_weight, nvox * sizeof(float)));
@@ -101,7 +148,6 @@ int main(int argc, char **argv) {
     HIPCHECK(hipMemcpy(d_K, cam_K, sizeof cam_K, hipMemcpyHostToDevice));
     CHECK(tdm_tsdf_init(nullptr, d_tsdf, d_weight, nvox, nullptr));
 
-    const int nf = (int)num_frames;
     const size_t px = (size_t)H * W;
     std::vector<float> depth(px);
     float *h_depth = nullptr, *h_pose = nullptr;  // pinned staging
@@ -135,20 +181,8 @@ int main(int argc, char **argv) {
                                    ox, oy, oz, voxel_size, trunc_margin, d_tsdf, d_weight, nullptr));
     }
 
-    std::vector<float> tsdf(nvox), weight(nvox);
     HIPCHECK(hipMemcpy(tsdf.data(), d_tsdf, nvox * sizeof(float), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(weight.data(), d_weight, nvox * sizeof(float), hipMemcpyDeviceToHost));
-    std::printf("Saving surface point cloud (%s)...\n", out_ply.c_str());
-    const float origin[3] = {ox, oy, oz};
-    int npts = 0;
-    CHECK(tdm_save_surface_ply(out_ply.c_str(), dims, voxel_size, origin, tsdf.data(), weight.data(),
-                               0.2f, 1.0f, &npts));
-    if (!tsdf_bin.empty()) {
-        std::printf("Saving TSDF voxel grid values to disk (%s)...\n", tsdf_bin.c_str());
-        CHECK(tdm_write_grid_bin(tsdf_bin.c_str(), dims, origin, voxel_size, trunc_margin, tsdf.data()));
-    }
-    if (!weight_bin.empty())
-        CHECK(tdm_write_grid_bin(weight_bin.c_str(), dims, origin, voxel_size, trunc_margin, weight.data()));
     (void)hipHostFree(h_depth);
     (void)hipHostFree(h_pose);
     (void)hipFree(d_depth);
@@ -156,5 +190,6 @@ int main(int argc, char **argv) {
     (void)hipFree(d_tsdf);
     (void)hipFree(d_weight);
     (void)hipFree(d_K);
-    return 0;
+    return write_outputs(out_ply, tsdf_bin, weight_bin, dims, origin, voxel_size, trunc_margin, tsdf,
+                         weight);
 }

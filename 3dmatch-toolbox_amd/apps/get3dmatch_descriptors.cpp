@@ -7,8 +7,10 @@
 //       grid [dx, dy, dz, grid] + keypoints [N, N*3 grid coordinates, rounded]
 //       -> <keypoints without .bin>.3dmatch.descriptors.bin [N, 512, ...].
 // Options: --net JSON (default tmp.json), --weights MARVIN (default
-// 3dmatch-weights-snapshot-137000.marvin), --precision fp32|bf16, --batch B.
+// 3dmatch-weights-snapshot-137000.marvin), --precision fp32|fp16|bf16|fp8
+// (default fp32; fp16 is the reference build's half storage), --batch B.
 // Every volume is processed (the reference drops the N mod 50 remainder).
+// Volumes must be patch^3 with the net's patch extent (30).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,24 +19,8 @@
 #include <string>
 #include <vector>
 
+#include "cli_common.hpp"
 #include "tdm.h"
-
-#define CHECK(call)                                                                        \
-    do {                                                                                   \
-        int s_ = (call);                                                                   \
-        if (s_ != TDM_OK) {                                                                \
-            std::fprintf(stderr, "error: %s: %s\n", tdm_status_string(s_), tdm_last_error()); \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
-#define HIPCHECK(call)                                                                     \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            std::fprintf(stderr, "HIP error: %s\n", hipGetErrorString(e_));                \
-            return 1;                                                                      \
-        }                                                                                  \
-    } while (0)
 
 static int read_floats(const char *path, std::vector<float> &v) {
     FILE *f = std::fopen(path, "rb");
@@ -59,7 +45,13 @@ int main(int argc, char **argv) {
         else if (a == "--batch" && i + 1 < argc) batch = std::atoi(argv[++i]);
         else if (a == "--precision" && i + 1 < argc) {
             std::string p = argv[++i];
-            precision = p == "bf16" ? TDM_PREC_BF16 : TDM_PREC_FP32;
+            if (!parse_precision(p, &precision)) {
+                std::fprintf(stderr, "unknown precision %s (fp32|fp16|bf16|fp8)\n", p.c_str());
+                return 1;
+            }
+        } else if (a.size() > 2 && a.compare(0, 2, "--") == 0) {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 1;
         } else pos.push_back(a);
     }
     if (pos.empty() || pos.size() > 2) {
@@ -77,11 +69,19 @@ int main(int argc, char **argv) {
         int n = 0, dims[3];
         CHECK(tdm_read_patch_volumes(pos[0].c_str(), &data, &n, dims));
         std::printf("%d\n", n);
+        const int P = net_patch_extent(net);
+        if (dims[0] != P || dims[1] != P || dims[2] != P) {
+            std::fprintf(stderr, "%s holds %dx%dx%d volumes; the net takes %d^3 patches\n",
+                         pos[0].c_str(), dims[0], dims[1], dims[2], P);
+            return 1;
+        }
         const size_t per = (size_t)dims[0] * dims[1] * dims[2];
         float *d_data = nullptr, *d_feat = nullptr;
         HIPCHECK(hipMalloc(&d_data, sizeof(float) * per * n + 4));
         HIPCHECK(hipMalloc(&d_feat, sizeof(float) * (size_t)n * D + 4));
         HIPCHECK(hipMemcpy(d_data, data, sizeof(float) * per * n, hipMemcpyHostToDevice));
+        if (precision == TDM_PREC_FP8 && n > 0)
+            CHECK(calibrate_fp8(net, net_json, weights, d_data, std::min(n, kCalibrationPatches)));
         CHECK(tdm_net_forward(net, d_data, n, d_feat, nullptr));
         std::vector<float> feat((size_t)n * D);
         HIPCHECK(hipMemcpy(feat.data(), d_feat, sizeof(float) * feat.size(), hipMemcpyDeviceToHost));
@@ -95,6 +95,11 @@ int main(int argc, char **argv) {
             std::fprintf(stderr, "cannot read %s\n", pos[0].c_str());
             return 1;
         }
+        for (int a = 0; a < 3; ++a)
+            if (!(g[a] >= 1.0f && g[a] <= 1e6f)) {
+                std::fprintf(stderr, "%s: bad grid dimension %g\n", pos[0].c_str(), g[a]);
+                return 1;
+            }
         const int dims[3] = {(int)g[0], (int)g[1], (int)g[2]};
         std::printf("%d %d %d\n", dims[0], dims[1], dims[2]);
         const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
@@ -104,6 +109,10 @@ int main(int argc, char **argv) {
         }
         if (read_floats(pos[1].c_str(), k) || k.empty()) {
             std::fprintf(stderr, "cannot read %s\n", pos[1].c_str());
+            return 1;
+        }
+        if (!(k[0] >= 0.0f && k[0] <= 1e8f)) {
+            std::fprintf(stderr, "%s: bad keypoint count %g\n", pos[1].c_str(), k[0]);
             return 1;
         }
         const int n = (int)k[0];
@@ -121,6 +130,15 @@ int main(int argc, char **argv) {
         HIPCHECK(hipMalloc(&d_feat, sizeof(float) * (size_t)n * D + 4));
         HIPCHECK(hipMemcpy(d_tdf, g.data() + 3, sizeof(float) * nvox, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(d_kp, kp.data(), sizeof(int) * kp.size(), hipMemcpyHostToDevice));
+        if (precision == TDM_PREC_FP8 && n > 0) {
+            const int nc = std::min(n, kCalibrationPatches), P = net_patch_extent(net);
+            float *d_cal = nullptr;
+            HIPCHECK(hipMalloc(&d_cal, sizeof(float) * (size_t)nc * P * P * P));
+            CHECK(tdm_extract_patches(nullptr, d_tdf, dims[0], dims[1], dims[2], d_kp, nc, P, d_cal,
+                                      nullptr));
+            CHECK(calibrate_fp8(net, net_json, weights, d_cal, nc));
+            (void)hipFree(d_cal);
+        }
         CHECK(tdm_net_forward_grid(net, d_tdf, dims[0], dims[1], dims[2], d_kp, n, d_feat, nullptr));
         std::vector<float> feat((size_t)n * D);
         HIPCHECK(hipMemcpy(feat.data(), d_feat, sizeof(float) * feat.size(), hipMemcpyDeviceToHost));

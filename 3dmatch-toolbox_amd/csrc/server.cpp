@@ -559,7 +559,7 @@ extern "C" {
 void tdm_server_config_init(tdm_server_config *cfg) {
     if (!cfg) return;
     cfg->memory_budget = (size_t)4 << 30;
-    cfg->max_inflight = 8;
+    cfg->max_inflight = 3;  // 3 request streams + the net's = HIP's 4 hardware queues
     cfg->keypoints = 5000;
     cfg->max_pass_keypoints = 20000;
     cfg->voxel_size = 0.01f;

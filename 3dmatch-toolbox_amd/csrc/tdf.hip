@@ -726,10 +726,16 @@ struct BinTable {
 
 // Decodes this thread's kBinPts points (tile index or kScatEmpty) and counts
 // them into the table; slot and rank per point.  Every thread calls it.
+// A point whose voxel lies outside the grid but within `margin` voxels of it
+// can still be the nearest point of a border voxel (the reference searches
+// every point it is given, training/match.hpp:40-48): it is binned into the
+// tile of its voxel clamped to the grid.  Clamping moves the voxel no
+// farther, per axis, from any voxel of the grid, so every tile whose walk
+// reaches the true voxel also reaches the clamped one.
 __device__ inline void bin_points(BinTable &T, const float *__restrict__ pts, int n, int dx,
-                                  int dy, int dz, float ox, float oy, float oz, float vs, int tx,
-                                  int ty, float4 (&p)[kBinPts], int (&slot)[kBinPts],
-                                  unsigned (&rank)[kBinPts]) {
+                                  int dy, int dz, float ox, float oy, float oz, float vs,
+                                  int margin, int tx, int ty, float4 (&p)[kBinPts],
+                                  int (&slot)[kBinPts], unsigned (&rank)[kBinPts]) {
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads) {
         T.key[i] = kScatEmpty;
         T.cnt[i] = 0u;
@@ -747,9 +753,14 @@ __device__ inline void bin_points(BinTable &T, const float *__restrict__ pts, in
         const float fx = roundf((p[k].x - ox) / vs);
         const float fy = roundf((p[k].y - oy) / vs);
         const float fz = roundf((p[k].z - oz) / vs);
-        const bool ok = base + (long)k * kBinThreads < n && fx >= 0.0f && fx < (float)dx &&
-                        fy >= 0.0f && fy < (float)dy && fz >= 0.0f && fz < (float)dz;
-        key[k] = ok ? (unsigned)acc_tile_of((int)fx, (int)fy, (int)fz, tx, ty) : kScatEmpty;
+        const float m = (float)margin;
+        const bool ok = base + (long)k * kBinThreads < n && fx >= -m && fx <= (float)(dx - 1) + m &&
+                        fy >= -m && fy <= (float)(dy - 1) + m && fz >= -m &&
+                        fz <= (float)(dz - 1) + m;
+        const int gx = (int)fminf(fmaxf(fx, 0.0f), (float)(dx - 1));
+        const int gy = (int)fminf(fmaxf(fy, 0.0f), (float)(dy - 1));
+        const int gz = (int)fminf(fmaxf(fz, 0.0f), (float)(dz - 1));
+        key[k] = ok ? (unsigned)acc_tile_of(gx, gy, gz, tx, ty) : kScatEmpty;
     }
     __syncthreads();
 #pragma unroll
@@ -772,12 +783,12 @@ __device__ inline void bin_points(BinTable &T, const float *__restrict__ pts, in
 
 __global__ __launch_bounds__(kBinThreads) void acc_count_kernel(
     const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
-    float vs, int tx, int ty, int *__restrict__ counts) {
+    float vs, int margin, int tx, int ty, int *__restrict__ counts) {
     __shared__ BinTable T;
     float4 p[kBinPts];
     int slot[kBinPts];
     unsigned rank[kBinPts];
-    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, tx, ty, p, slot, rank);
+    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, margin, tx, ty, p, slot, rank);
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads)
         if (T.key[i] != kScatEmpty) atomicAdd(&counts[T.key[i]], (int)T.cnt[i]);
 }
@@ -835,13 +846,13 @@ __global__ __launch_bounds__(1024) void acc_offsets_kernel(const int *__restrict
 // kernel below takes a minimum).
 __global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
     const float *__restrict__ pts, int n, int dx, int dy, int dz, float ox, float oy, float oz,
-    float vs, int tx, int ty, const int *__restrict__ offsets, int *__restrict__ cursor,
-    float4 *__restrict__ binned) {
+    float vs, int margin, int tx, int ty, const int *__restrict__ offsets,
+    int *__restrict__ cursor, float4 *__restrict__ binned) {
     __shared__ BinTable T;
     float4 p[kBinPts];
     int slot[kBinPts];
     unsigned rank[kBinPts];
-    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, tx, ty, p, slot, rank);
+    bin_points(T, pts, n, dx, dy, dz, ox, oy, oz, vs, margin, tx, ty, p, slot, rank);
     for (int i = threadIdx.x; i < kBinSlots; i += kBinThreads)
         if (T.key[i] != kScatEmpty)
             T.cnt[i] = (unsigned)offsets[T.key[i]] + (unsigned)atomicAdd(&cursor[T.key[i]], (int)T.cnt[i]);
@@ -997,19 +1008,69 @@ int tdf_work(tdm_ctx *ctx, int decode, size_t nvox, float *tdf, hipStream_t s, f
     return TDM_OK;
 }
 
-// Bit grid and, after it, the per-tile flags, in scratch slot 0; one launch
-// clears both and zeroes the output grid tdf (dx*dy*dz floats; none if null).
-int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, float *tdf, unsigned **bits,
-                 size_t *words, unsigned char **flags, hipStream_t s) {
+// Bytes of a bit grid plus its per-tile flags.
+size_t bits_bytes(int dx, int dy, int dz) {
+    return (size_t)row_words(dx) * dy * dz * sizeof(unsigned) + num_tiles(dx, dy, dz);
+}
+
+// Bit grid and, after it, the per-tile flags at p (bits_bytes, 16-byte
+// aligned); one launch clears both and zeroes the output grid tdf (dx*dy*dz
+// floats; none if null).
+int clear_bits(void *p, int dx, int dy, int dz, float *tdf, unsigned **bits, size_t *words,
+               unsigned char **flags, hipStream_t s) {
     *words = (size_t)row_words(dx) * dy * dz;
-    const size_t bytes = *words * sizeof(unsigned) + num_tiles(dx, dy, dz);
-    void *p = nullptr;
-    TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &p, s));
     *bits = static_cast<unsigned *>(p);
     *flags = reinterpret_cast<unsigned char *>(*bits + *words);
-    clear_regions(tdf, (size_t)dx * dy * dz, static_cast<unsigned char *>(p), bytes, s);
+    clear_regions(tdf, (size_t)dx * dy * dz, static_cast<unsigned char *>(p), bits_bytes(dx, dy, dz),
+                  s);
     TDM_LAUNCHED();
     return TDM_OK;
+}
+
+// The same with the bit grid in the context's scratch slot 0.
+int bits_scratch(tdm_ctx *ctx, int dx, int dy, int dz, float *tdf, unsigned **bits,
+                 size_t *words, unsigned char **flags, hipStream_t s) {
+    void *p = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, bits_bytes(dx, dy, dz), &p, s));
+    return clear_bits(p, dx, dy, dz, tdf, bits, words, flags, s);
+}
+
+// core/demo.cu:127-158 from points, given the bit grid region p (bits_bytes):
+// clear it, scatter the points (the deduplicating scatter zeroes `work` in
+// the same launch), then the tile pass; `work` and `tdf` as in tdf_from_bits.
+int points_to_tdf(void *p, const float *pts, int num_pts, const int dims[3], const float origin[3],
+                  float voxel_size, int r, int decode, float *work, float *tdf, hipStream_t s) {
+    unsigned *bits = nullptr;
+    size_t words = 0;
+    unsigned char *flags = nullptr;
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    const bool dedup = num_pts > 0 && (size_t)row_words(dims[0]) * dims[1] * dims[2] < kScatEmpty;
+    TDM_TRY(clear_bits(p, dims[0], dims[1], dims[2], dedup ? nullptr : work, &bits, &words, &flags,
+                       s));
+    if (dedup) {
+        const long per_block = (long)kScatThreads * kScatPts;
+        const unsigned nscat = (unsigned)((num_pts + per_block - 1) / per_block);
+        const ZeroJob z = zero_job(work, nvox);
+        const unsigned nfill = (unsigned)std::min<size_t>(z.n4 / kScatThreads + 1, 512);
+        scatter_bits_dedup_kernel<<<nscat + nfill, kScatThreads, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            bits, flags, z, nscat);
+        TDM_LAUNCHED();
+    } else if (num_pts > 0) {
+        scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
+            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
+            bits, flags);
+        TDM_LAUNCHED();
+    }
+    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, work, tdf, flags, true, s);
+}
+
+int check_points_args(const float *pts, int num_pts, const int dims[3], const float origin[3],
+                      float *tdf) {
+    TDM_CHECK((pts || num_pts == 0) && dims && origin && tdf, TDM_ERR_INVALID_ARG,
+              "null pointer argument");
+    TDM_CHECK(num_pts >= 0, TDM_ERR_INVALID_ARG, "negative point count");
+    return check_dims(dims[0], dims[1], dims[2]);
 }
 
 }  // namespace
@@ -1112,42 +1173,19 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
                         const float origin[3], float voxel_size, float trunc_margin, int decode,
                         float *tdf, void *stream) {
     tdm::clear_error();
-    TDM_CHECK((pts || num_pts == 0) && dims && origin && tdf, TDM_ERR_INVALID_ARG,
-              "null pointer argument");
-    TDM_CHECK(num_pts >= 0, TDM_ERR_INVALID_ARG, "negative point count");
+    TDM_TRY(check_points_args(pts, num_pts, dims, origin, tdf));
     TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
               "bad decode mode %d", decode);
-    TDM_TRY(check_dims(dims[0], dims[1], dims[2]));
     int r = 0;
     TDM_TRY(search_radius(voxel_size, trunc_margin, &r));
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     hipStream_t s = tdm::as_stream(stream);
-    unsigned *bits = nullptr;
-    size_t words = 0;
-    unsigned char *flags = nullptr;
     const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
     float *work = nullptr;
     TDM_TRY(tdf_work(ctx, decode, nvox, tdf, s, &work));
-    const bool dedup = num_pts > 0 && (size_t)row_words(dims[0]) * dims[1] * dims[2] < kScatEmpty;
-    // the deduplicating scatter zeroes the output grid in its own launch
-    TDM_TRY(bits_scratch(ctx, dims[0], dims[1], dims[2], dedup ? nullptr : work, &bits, &words,
-                         &flags, s));
-    if (dedup) {
-        const long per_block = (long)kScatThreads * kScatPts;
-        const unsigned nscat = (unsigned)((num_pts + per_block - 1) / per_block);
-        const ZeroJob z = zero_job(work, nvox);
-        const unsigned nfill = (unsigned)std::min<size_t>(z.n4 / kScatThreads + 1, 512);
-        scatter_bits_dedup_kernel<<<nscat + nfill, kScatThreads, 0, s>>>(
-            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
-            bits, flags, z, nscat);
-        TDM_LAUNCHED();
-    } else if (num_pts > 0) {
-        scatter_bits_kernel<<<tdm::blocks_for(num_pts, 256), 256, 0, s>>>(
-            pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
-            bits, flags);
-        TDM_LAUNCHED();
-    }
-    return tdf_from_bits(bits, dims[0], dims[1], dims[2], r, decode, work, tdf, flags, true, s);
+    void *p = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, bits_bytes(dims[0], dims[1], dims[2]), &p, s));
+    return points_to_tdf(p, pts, num_pts, dims, origin, voxel_size, r, decode, work, tdf, s);
 }
 
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
@@ -1202,6 +1240,8 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     const int reach = (int)std::ceil(reach_vox / kAT);
     TDM_CHECK(reach <= 4, TDM_ERR_UNSUPPORTED, "truncation of %g voxels is too wide",
               (double)trunc_margin / voxel_size);
+    // points binned up to this many voxels outside the grid (>= reach_vox)
+    const int margin = reach * kAT;
     const long nb = (nt + 1023) / 1024;  // scan blocks
     const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1 + nb) + 16 + sizeof(float4) * (size_t)num_pts;
     void *scratch = nullptr;
@@ -1218,7 +1258,7 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
         acc_count_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
                            kBinThreads, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
-            tx, ty, counts);
+            margin, tx, ty, counts);
         TDM_LAUNCHED();
     }
     acc_blocksum_kernel<<<(unsigned)nb, 1024, 0, s>>>(counts, nt, bsum);
@@ -1231,7 +1271,7 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
         acc_fill_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
                           kBinThreads, 0, s>>>(
             pts, num_pts, dims[0], dims[1], dims[2], origin[0], origin[1], origin[2], voxel_size,
-            tx, ty, offsets, counts, binned);
+            margin, tx, ty, offsets, counts, binned);
         TDM_LAUNCHED();
     }
     TDM_CHECK(nt < (1L << 31), TDM_ERR_INVALID_ARG, "grid too large");

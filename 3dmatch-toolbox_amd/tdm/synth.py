@@ -248,3 +248,24 @@ def depth_frames_torch(torch, n: int = 50, scene_seed: int = 1, noise_seed: int 
         m = (mm.float() / 1000.0)
         out[i] = torch.where(m > 6.0, torch.zeros_like(m), m)
     return out, poses
+
+
+def write_png16(path: str, mm: np.ndarray) -> None:
+    """16-bit grayscale PNG (filter 0 on every row) of a [H, W] uint16 image,
+    the format of the reference's frame-%06d.depth.png files."""
+    import struct
+    import zlib
+
+    h, w = mm.shape
+    raw = b"".join(b"\x00" + mm[r].astype(">u2").tobytes() for r in range(h))
+
+    def chunk(tag, data):
+        c = struct.pack(">I", len(data)) + tag + data
+        return c + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n")
+        f.write(chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 16, 0, 0, 0, 0)))
+        f.write(chunk(b"IDAT", zlib.compress(raw, 6)))
+        f.write(chunk(b"IEND", b""))
+

@@ -195,16 +195,19 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
  * deprecated/pointCloud2AccTDF.m): tdf = 1 - min(d, trunc_margin) /
  * trunc_margin with d the Euclidean distance from each voxel centre
  * (origin + index * voxel_size) to the nearest point, in float: d^2 summed
- * as (dx^2 + dy^2) + dz^2, sqrtf correctly rounded.  Points outside the
- * grid are ignored.  pts, tdf: device.  Reach up to 31.5 voxels. */
+ * as (dx^2 + dy^2) + dz^2, sqrtf correctly rounded.  Every point counts,
+ * inside the grid or not, as the reference searches all the points it is
+ * given.  pts, tdf: device.  Reach up to 31.5 voxels. */
 int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims[3],
                      const float origin[3], float voxel_size, float trunc_margin, float *tdf,
                      void *stream);
 
 /* Many clouds in one launch per step: the TDF of core/demo.cu:127-158 for a
  * batch of fragments of different sizes, as a plan (made once, on the host)
- * and runs (three launches for the whole batch: clear the bit grids, scatter
- * every cloud's points, one tile kernel over every grid's tiles).  A run
+ * and runs (four launches for the whole batch: zero every cloud's output
+ * grid, clear the bit grids, scatter every cloud's points, one tile kernel
+ * over every grid's tiles; in reference decode one more launch per cloud
+ * rewrites the voxels whose float32 decode differs).  A run
  * allocates nothing, sets no memory with memset and never waits on the host,
  * so it can be captured into a hipGraph and replayed; it reads the point
  * buffers named in the plan at run time.  Results are bit-identical to
@@ -417,7 +420,9 @@ typedef struct tdm_server_config {
     long ransac_max_trials;   /* ransac.m:121: 50000                          */
 } tdm_server_config;
 
-/* The defaults above; memory_budget 4 GiB, 8 slots, 5000 keypoints, passes
+/* The defaults above; memory_budget 4 GiB, 3 slots (3 request streams and
+ * the net's stream fill the 4 hardware queues HIP gives a process by
+ * default), 5000 keypoints, passes
  * of up to 20000 keypoints. */
 void tdm_server_config_init(tdm_server_config *cfg);
 

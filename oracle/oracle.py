@@ -256,19 +256,17 @@ def surface_points(tsdf: np.ndarray, weight: np.ndarray, origin, voxel_size, tsd
 def tdf_accurate(pts, dims, origin, voxel_size, trunc) -> np.ndarray:
     """Get1NNSearchDist (training/match.hpp:13-51) on the demo grid: for each
     voxel centre origin + idx*vs (float32), d = sqrtf((dx^2 + dy^2) + dz^2)
-    to the nearest point in float32, tdf = 1 - min(d, trunc)/trunc.  Points
-    outside the grid (by the occupancy rounding) are ignored.  The candidates
-    of each voxel come from a float64 KD-tree query with a margin; the
-    distances themselves are evaluated in float32 in the reference's order."""
+    to the nearest point in float32, tdf = 1 - min(d, trunc)/trunc.  Every
+    point counts, inside the grid or not (the kernel loops over all the
+    points it is given, training/match.hpp:40-48).  The candidates of each
+    voxel come from a float64 KD-tree query with a margin; the distances
+    themselves are evaluated in float32 in the reference's order."""
     from scipy.spatial import cKDTree
 
     pts = np.asarray(pts, np.float32)
     dx, dy, dz = dims
     o = np.asarray(origin, np.float32)
     vs, tr = np.float32(voxel_size), np.float32(trunc)
-    g = np.round((pts - o) / vs)
-    inside = np.all((g >= 0) & (g < np.array([dx, dy, dz])), axis=1)
-    pts = pts[inside]
     z, y, x = np.meshgrid(np.arange(dz), np.arange(dy), np.arange(dx), indexing="ij")
     c = np.stack([o[0] + x.astype(np.float32) * vs, o[1] + y.astype(np.float32) * vs,
                   o[2] + z.astype(np.float32) * vs], -1).reshape(-1, 3)

@@ -141,3 +141,36 @@ def test_unsupported_layers_rejected(tmp_path):
 def test_flops_per_patch():
     assert netdef.flops_per_patch() == 6_452_240_384
     assert sum(a.size for _, a in netdef.he_normal_weights(5)) == 14_311_488
+
+
+def _cli(args, cwd):
+    import subprocess
+
+    return subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+@pytest.fixture(scope="module")
+def cli_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli_args")
+    netdef.write_net_json(str(d / "tmp.json"))
+    netdef.write_net_json(str(d / "3dmatch-net-test.json"))
+    tdm.marvin_write(str(d / "3dmatch-weights-snapshot-137000.marvin"), netdef.he_normal_weights(5))
+    return d
+
+
+@pytest.mark.parametrize("tool", ["get3dmatch_descriptors", "descriptor_demo"])
+def test_cli_rejects_unknown_precision(cli_dir, tool):
+    args = [os.path.join(tdm.BIN_DIR, tool)]
+    args += ["data.bin"] if tool == "get3dmatch_descriptors" else ["cloud.ply", "out"]
+    r = _cli(args + ["--precision", "fp64"], str(cli_dir))
+    assert r.returncode != 0 and "unknown precision fp64" in r.stderr
+
+
+def test_get3dmatch_descriptors_rejects_wrong_volume_size(cli_dir):
+    """data.bin volumes must have the net's 30^3 extent: 20^3 volumes are
+    refused after reading the file, before any device buffer or launch."""
+    vols = np.zeros((2, 20, 20, 20), np.float32)
+    (cli_dir / "small.bin").write_bytes(struct.pack("<4f", 2, 20, 20, 20) + vols.tobytes())
+    r = _cli([os.path.join(tdm.BIN_DIR, "get3dmatch_descriptors"), "small.bin"], str(cli_dir))
+    assert r.returncode != 0 and "30^3" in r.stderr, r.stderr
+    assert not (cli_dir / "feat.bin").exists()

@@ -66,12 +66,14 @@ def test_fusion_demo_matches_oracle(torch_gpu, tmp_path):
     want = np.stack([origin[0] + x.astype(np.float32) * vs, origin[1] + y.astype(np.float32) * vs,
                      origin[2] + z.astype(np.float32) * vs], 1)
     assert len(pts) > 1000 and np.array_equal(pts, want)
-    # fused mode writes the same files
-    run([os.path.join(BIN, "fusion_demo"), os.path.join(frames, "camera-intrinsics.txt"), frames,
-         "8", "8", "5", "-1.5", "-1.5", "0.5", "0.012", str(float(trunc)), "--dims", "200", "200",
-         "200", "--tsdf-bin", "tsdf2.bin", "--fused", "--out", "tsdf2.ply"], str(tmp_path))
-    assert open(tmp_path / "tsdf2.bin", "rb").read() == open(tmp_path / "tsdf.bin", "rb").read()
-    assert open(tmp_path / "tsdf2.ply", "rb").read() == open(tmp_path / "tsdf.ply", "rb").read()
+    # the serial loop and the fused pass write the same files as the default
+    # overlapped pipeline
+    for k, mode in enumerate(("--fused", "--serial"), start=2):
+        run([os.path.join(BIN, "fusion_demo"), os.path.join(frames, "camera-intrinsics.txt"), frames,
+             "8", "8", "5", "-1.5", "-1.5", "0.5", "0.012", str(float(trunc)), "--dims", "200", "200",
+             "200", "--tsdf-bin", f"tsdf{k}.bin", mode, "--out", f"tsdf{k}.ply"], str(tmp_path))
+        assert open(tmp_path / f"tsdf{k}.bin", "rb").read() == open(tmp_path / "tsdf.bin", "rb").read()
+        assert open(tmp_path / f"tsdf{k}.ply", "rb").read() == open(tmp_path / "tsdf.ply", "rb").read()
 
 
 def test_descriptor_demo_matches_oracle(torch_gpu, net_dir, tmp_path):
@@ -97,6 +99,20 @@ def test_descriptor_demo_matches_oracle(torch_gpu, net_dir, tmp_path):
     desc = tdm.read_desc_bin(str(tmp_path / "frag.desc.3dmatch.bin"))
     assert desc.shape == (7, 512)
     assert rel_l2(desc, ref) <= 1e-5
+    # --precision fp16 (the reference build's half storage) gives what the
+    # library's fp16 net gives on the same grid and keypoints, bit for bit
+    run([os.path.join(BIN, "descriptor_demo"), "cloud.ply", "h", "--keypoint-indices", "kp.txt",
+         "--precision", "fp16"], str(tmp_path))
+    half = tdm.read_desc_bin(str(tmp_path / "h.desc.3dmatch.bin"))
+    net = tdm.Net(str(d / "3dmatch-net-test.json"), tdm.PREC_FP16)
+    net.load_weights(str(d / "3dmatch-weights-snapshot-137000.marvin"))
+    feat = torch_gpu.empty(7, 512, device="cuda:0")
+    kpg = oracle.keypoint_grid(kp, origin, vs).astype(np.int32)
+    net.forward_grid(torch_gpu.from_numpy(tdf).to("cuda:0"), dims,
+                     torch_gpu.from_numpy(kpg).to("cuda:0"), feat)
+    assert np.array_equal(half.view(np.uint32), feat.cpu().numpy().view(np.uint32))
+    assert rel_l2(half, ref) <= 1e-2
+    net.close()
     # seeded draws are reproducible
     run([os.path.join(BIN, "descriptor_demo"), "cloud.ply", "s1", "--seed", "7", "--num-keypoints", "50"], str(tmp_path))
     run([os.path.join(BIN, "descriptor_demo"), "cloud.ply", "s2", "--seed", "7", "--num-keypoints", "50"], str(tmp_path))
@@ -120,10 +136,34 @@ def test_get3dmatch_descriptors_both_forms(torch_gpu, net_dir, tmp_path):
     # shape2pose form: TDF grid with a 3-float header + grid-coordinate keypoints
     grid = rng.random((40, 45, 50), dtype=np.float32)
     open(tmp_path / "TDF.bin", "wb").write(struct.pack("<3f", 50, 45, 40) + grid.tobytes())
-    kpg = np.array([[20.2, 22.7, 19.5], [30, 25, 20], [15, 15, 15]], np.float32)
-    open(tmp_path / "kp.bin", "wb").write(struct.pack("<f", 3) + kpg.tobytes())
+    # 20.5 and 24.5 round away from zero (C round, as the reference's
+    # get3DMatchDescriptors.cu:49-51), where half-to-even would give 20 and 24
+    kpg = np.array([[20.2, 22.7, 19.5], [30, 25, 20], [15, 15, 15], [20.5, 24.5, 17.5]], np.float32)
+    open(tmp_path / "kp.bin", "wb").write(struct.pack("<f", len(kpg)) + kpg.tobytes())
     run([os.path.join(BIN, "get3dmatch_descriptors"), "TDF.bin", "kp.bin"], str(tmp_path))
     desc = tdm.read_desc_bin(str(tmp_path / "kp.3dmatch.descriptors.bin"))
-    patches = oracle.extract_patches(grid, np.round(kpg).astype(np.int32))
-    ref = oracle.net_forward_fp64(patches, w, netdef.layer_shapes())[-1].reshape(3, 512)
+    kpi = (np.sign(kpg) * np.floor(np.abs(kpg) + np.float32(0.5))).astype(np.int32)
+    assert kpi[3].tolist() == [21, 25, 18]
+    patches = oracle.extract_patches(grid, kpi)
+    ref = oracle.net_forward_fp64(patches, w, netdef.layer_shapes())[-1].reshape(len(kpg), 512)
     assert rel_l2(desc, ref) <= 1e-5
+    # a keypoint rounded half to even instead would read a patch one voxel off
+    off = oracle.net_forward_fp64(oracle.extract_patches(grid, np.round(kpg[3:]).astype(np.int32)), w,
+                                  netdef.layer_shapes())[-1].reshape(1, 512)
+    assert rel_l2(desc[3:], off) > 1e-3
+
+    # fp16 and fp8 are selectable; fp16 equals the library's fp16 net bit for bit
+    for prec, tol in (("fp16", 1e-2), ("fp8", 0.15)):
+        run([os.path.join(BIN, "get3dmatch_descriptors"), "data.bin", "--precision", prec],
+            str(tmp_path))
+        raw = open(tmp_path / "feat.bin", "rb").read()
+        got = np.frombuffer(raw[20:], np.float32).reshape(53, 512)
+        assert rel_l2(got[:4], oracle.net_forward_fp64(vols[:4], w, netdef.layer_shapes())[-1]
+                      .reshape(4, 512)) <= tol
+        if prec == "fp16":
+            net = tdm.Net(str(tmp_path / "tmp.json"), tdm.PREC_FP16)
+            net.load_weights(str(tmp_path / "3dmatch-weights-snapshot-137000.marvin"))
+            f16 = torch_gpu.empty(53, 512, device="cuda:0")
+            net.forward(torch_gpu.from_numpy(vols).to("cuda:0"), f16)
+            assert np.array_equal(got.view(np.uint32), f16.cpu().numpy().view(np.uint32))
+            net.close()

@@ -192,8 +192,9 @@ def test_accurate_tdf_bit_exact(torch_gpu):
     cloud = synth.fragment(8_000, view=3, seed=77)
     vs, tr = np.float32(0.01), np.float32(0.05)
     full, origin = oracle.grid_params(cloud, vs, 15)
-    # a 90^3 sub-block around the cloud's median keeps the oracle fast (points
-    # outside it are dropped by both); shift the origin by whole voxels
+    # a 90^3 sub-block around the cloud's median keeps the oracle fast; it
+    # cuts through the cloud, so points just outside it are the nearest points
+    # of its border voxels; shift the origin by whole voxels
     med = np.median(cloud, axis=0)
     off = np.clip(np.round((med - origin) / vs).astype(int) - 45, 0, None)
     origin = (origin + off.astype(np.float32) * vs).astype(np.float32)
@@ -206,3 +207,8 @@ def test_accurate_tdf_bit_exact(torch_gpu):
     g = got.cpu().numpy()
     assert np.count_nonzero(want) > 1000
     assert np.array_equal(g.view(np.uint32), want.view(np.uint32)), np.abs(g - want).max()
+    # the cut matters: dropping the outside points changes border voxels
+    gi = np.round((cloud - origin) / vs)
+    inside = np.all((gi >= 0) & (gi < np.array(dims)), axis=1)
+    assert 0 < inside.sum() < len(cloud)
+    assert not np.array_equal(oracle.tdf_accurate(cloud[inside], dims, origin, vs, tr), want)

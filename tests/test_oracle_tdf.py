@@ -126,19 +126,25 @@ def test_accurate_tdf_oracle_vs_brute_force():
     dims = (14, 11, 9)
     origin = np.array([0.1, -0.2, 0.3], np.float32)
     vs, tr = np.float32(0.01), np.float32(0.05)
-    pts = (origin + rng.uniform(0, 1, (40, 3)) * np.array(dims) * vs).astype(np.float32)
+    # points up to 4 voxels beyond the grid on every side count too
+    lo, hi = -4.5, np.array(dims) + 3.5
+    pts = (origin + rng.uniform(lo, hi, (60, 3)) * vs).astype(np.float32)
     pts[0] = origin + np.float32(3) * vs  # exactly on the centre of voxel (3, 3, 3)
+    pts[1] = origin + np.array([-2.2, 5, 4], np.float32) * vs  # outside, 2.2 voxels from x = 0
     got = oracle.tdf_accurate(pts, dims, origin, vs, tr)
     z, y, x = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
     c = np.stack([origin[0] + x.astype(np.float32) * vs, origin[1] + y.astype(np.float32) * vs,
                   origin[2] + z.astype(np.float32) * vs], -1).reshape(-1, 3)
-    g = np.round((pts - origin) / vs)
-    keep = np.all((g >= 0) & (g < np.array(dims)), axis=1)
-    e = c[:, None, :] - pts[None, keep, :]
+    e = c[:, None, :] - pts[None, :, :]
     d2 = (e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]
     d = np.sqrt(d2.min(1))
     want = (np.float32(1) - np.minimum(d, tr) / tr).reshape(dims[2], dims[1], dims[0])
     assert np.array_equal(got, want)
     assert got[3, 3, 3] == 1.0
+    # voxel (0, 5, 4) is 2.2 voxels from the outside point: 1 - 2.2/5 or more
+    assert got[4, 5, 0] >= np.float32(1) - np.float32(0.022) / tr - np.float32(1e-6)
+    g = np.round((pts - origin) / vs)
+    outside = ~np.all((g >= 0) & (g < np.array(dims)), axis=1)
+    assert outside.sum() > 10
     far = d > tr
     assert np.all(got.reshape(-1)[far] == 0.0)
