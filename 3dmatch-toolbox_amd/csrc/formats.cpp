@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -50,6 +51,33 @@ int read_count(FILE *f, const char *path, int *n) {
     TDM_CHECK(std::isfinite(v) && v >= 0 && v <= 2.0e9f && v == std::floor(v), TDM_ERR_FORMAT,
               "%s: bad count header %g", path, (double)v);
     *n = (int)v;
+    return TDM_OK;
+}
+
+// Bytes from the current position of f to its end; -1 when the stream
+// cannot seek (then the read itself reports a short file).
+long long bytes_left(FILE *f) {
+    const long pos = std::ftell(f);
+    if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+    const long end = std::ftell(f);
+    if (std::fseek(f, pos, SEEK_SET) != 0 || end < pos) return -1;
+    return (long long)(end - pos);
+}
+
+// The element count a header declares (the product of `factors`), checked
+// for overflow and against what the file still holds before anything is
+// allocated: a hostile header must not turn into a huge allocation.
+int payload_count(long long left, const char *path, std::initializer_list<size_t> factors,
+                  size_t elem, size_t *count) {
+    size_t c = 1;
+    for (size_t v : factors) {
+        TDM_CHECK(v == 0 || c <= SIZE_MAX / elem / v, TDM_ERR_FORMAT,
+                  "%s: header sizes overflow", path);
+        c *= v;
+    }
+    TDM_CHECK(left < 0 || (unsigned long long)left >= c * elem, TDM_ERR_FORMAT,
+              "%s is truncated: the header declares %zu bytes, %lld remain", path, c * elem, left);
+    *count = c;
     return TDM_OK;
 }
 
@@ -115,6 +143,9 @@ int decode_png_u16(const char *path, std::vector<uint16_t> &pix, int &H, int &W)
     TDM_CHECK(interlace == 0, TDM_ERR_UNSUPPORTED, "%s: interlaced PNG", path);
     TDM_CHECK((long)W * H <= (1L << 28), TDM_ERR_FORMAT, "%s: image too large", path);
     const size_t stride = (size_t)W * 2;
+    // deflate expands at most 1032:1, so a short IDAT cannot hold a large image
+    TDM_CHECK((stride + 1) * H <= idat.size() * 1032 + 1024, TDM_ERR_FORMAT,
+              "%s: image data too short for %d x %d", path, W, H);
     std::vector<unsigned char> raw((stride + 1) * H);
     z_stream zs;
     std::memset(&zs, 0, sizeof zs);
@@ -208,8 +239,21 @@ int tdm_read_ply_points(const char *path, float **pts, int *num) {
     }
     TDM_CHECK(n > 0 && n < (1L << 30), TDM_ERR_FORMAT,
               "%s: third header line does not give a point count", path);
+    long long left = -1;
+    {
+        const std::streamoff here = in.tellg();
+        if (here >= 0 && in.seekg(0, std::ios::end)) {
+            const std::streamoff end = in.tellg();
+            if (end >= here) left = (long long)(end - here);
+        }
+        in.clear();
+        in.seekg(here);
+        TDM_CHECK(in, TDM_ERR_IO, "%s: cannot seek", path);
+    }
+    size_t count = 0;
+    TDM_TRY(payload_count(left, path, {(size_t)n, 3}, sizeof(float), &count));
     float *p = nullptr;
-    TDM_TRY(alloc_copy((size_t)n * 3, &p));
+    TDM_TRY(alloc_copy(count, &p));
     in.read(reinterpret_cast<char *>(p), (std::streamsize)(sizeof(float) * 3 * n));
     if (in.gcount() != (std::streamsize)(sizeof(float) * 3 * n)) {
         std::free(p);
@@ -249,8 +293,10 @@ int tdm_read_keypts_bin(const char *path, float **kp, int *num) {
     TDM_CHECK(f.f, TDM_ERR_IO, "cannot open %s", path);
     int n = 0;
     TDM_TRY(read_count(f.f, path, &n));
+    size_t count = 0;
+    TDM_TRY(payload_count(bytes_left(f.f), path, {(size_t)n, 3}, sizeof(float), &count));
     float *p = nullptr;
-    TDM_TRY(alloc_copy((size_t)n * 3, &p));
+    TDM_TRY(alloc_copy(count, &p));
     int s = read_all(f.f, p, sizeof(float) * 3 * (size_t)n, path);
     if (s != TDM_OK) {
         std::free(p);
@@ -280,8 +326,10 @@ int tdm_read_desc_bin(const char *path, float **desc, int *num, int *dim) {
     int n = 0, d = 0;
     TDM_TRY(read_count(f.f, path, &n));
     TDM_TRY(read_count(f.f, path, &d));
+    size_t count = 0;
+    TDM_TRY(payload_count(bytes_left(f.f), path, {(size_t)n, (size_t)d}, sizeof(float), &count));
     float *p = nullptr;
-    TDM_TRY(alloc_copy((size_t)n * d, &p));
+    TDM_TRY(alloc_copy(count, &p));
     int s = read_all(f.f, p, sizeof(float) * (size_t)n * d, path);
     if (s != TDM_OK) {
         std::free(p);
@@ -323,7 +371,9 @@ int tdm_read_grid_bin(const char *path, int dims[3], float origin[3], float *vox
     }
     *voxel_size = h[6];
     *trunc_margin = h[7];
-    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    size_t n = 0;
+    TDM_TRY(payload_count(bytes_left(f.f), path,
+                          {(size_t)dims[0], (size_t)dims[1], (size_t)dims[2]}, sizeof(float), &n));
     float *p = nullptr;
     TDM_TRY(alloc_copy(n, &p));
     int s = read_all(f.f, p, sizeof(float) * n, path);
@@ -342,15 +392,17 @@ int tdm_read_patch_volumes(const char *path, float **data, int *num, int dims[3]
     TDM_CHECK(f.f, TDM_ERR_IO, "cannot open %s", path);
     int n = 0;
     TDM_TRY(read_count(f.f, path, &n));
-    size_t per = 1;
     for (int a = 0; a < 3; ++a) {
         TDM_TRY(read_count(f.f, path, &dims[a]));
         TDM_CHECK(dims[a] > 0, TDM_ERR_FORMAT, "%s: zero volume dimension", path);
-        per *= (size_t)dims[a];
     }
+    size_t count = 0;
+    TDM_TRY(payload_count(bytes_left(f.f), path,
+                          {(size_t)n, (size_t)dims[0], (size_t)dims[1], (size_t)dims[2]},
+                          sizeof(float), &count));
     float *p = nullptr;
-    TDM_TRY(alloc_copy(per * n, &p));
-    int s = read_all(f.f, p, sizeof(float) * per * n, path);
+    TDM_TRY(alloc_copy(count, &p));
+    int s = read_all(f.f, p, sizeof(float) * count, path);
     if (s != TDM_OK) {
         std::free(p);
         return s;

@@ -38,6 +38,14 @@ int marvin_read(const char *path, std::vector<MarvinTensor> &out) {
     out.clear();
     FILE *f = std::fopen(path, "rb");
     TDM_CHECK(f, TDM_ERR_IO, "cannot open weights file %s", path);
+    // the file size bounds every tensor, so a hostile header is refused
+    // before its data is allocated
+    long long file_bytes = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) file_bytes = std::ftell(f);
+    if (std::fseek(f, 0, SEEK_SET) != 0) {
+        std::fclose(f);
+        return fail(TDM_ERR_IO, "cannot seek in %s", path);
+    }
     int status = TDM_OK;
     while (true) {
         uint8_t type;
@@ -70,6 +78,11 @@ int marvin_read(const char *path, std::vector<MarvinTensor> &out) {
         if (bad || esize == 0 || esize > 16) {
             status = fail(TDM_ERR_FORMAT, "%s: tensor %s has bad dims or element size", path,
                           t.name.c_str());
+            break;
+        }
+        const long here = std::ftell(f);
+        if (file_bytes >= 0 && here >= 0 && (unsigned long long)(file_bytes - here) < n * esize) {
+            status = fail(TDM_ERR_FORMAT, "%s: tensor %s is truncated", path, t.name.c_str());
             break;
         }
         std::vector<unsigned char> raw(n * esize);

@@ -16,7 +16,8 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO_DIR = os.path.dirname(PKG_DIR)
-LIB_PATH = os.path.join(PKG_DIR, "lib3dmatch_amd.so")
+# TDM_LIB names another build of the library (the host sanitizer build)
+LIB_PATH = os.environ.get("TDM_LIB") or os.path.join(PKG_DIR, "lib3dmatch_amd.so")
 HEADER_PATH = os.path.join(REPO_DIR, "include", "tdm.h")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
 

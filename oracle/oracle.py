@@ -20,7 +20,8 @@ from ctypes import c_float, c_int, c_long, c_void_p
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# TDM_ORACLE_LIB names another build of the oracle (the sanitizer build)
+LIB_PATH = os.environ.get("TDM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _lib = None
 

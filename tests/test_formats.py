@@ -48,6 +48,15 @@ def test_sample_clouds_and_grid_dims():
         assert d == dims
 
 
+def test_sample_cloud_fixture():
+    """The committed copy of data/sample/3dmatch-demo/single-depth-2.ply reads
+    as the reference's demo would (7 header lines, N on line 3)."""
+    pts = tdm.read_ply_points(os.path.join(GOLDEN, "sample_clouds", "single-depth-2.ply"))
+    assert pts.shape == (284874, 3) and pts.dtype == np.float32
+    d, _ = oracle.grid_params(pts, np.float32(0.01), 15)
+    assert d == [299, 255, 263]
+
+
 def test_keypts_desc_feat_grid_layouts(tmp_path):
     rng = np.random.default_rng(1)
     kp = rng.random((5, 3), dtype=np.float32)

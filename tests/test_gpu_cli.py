@@ -167,3 +167,43 @@ def test_get3dmatch_descriptors_both_forms(torch_gpu, net_dir, tmp_path):
             net.forward(torch_gpu.from_numpy(vols).to("cuda:0"), f16)
             assert np.array_equal(got.view(np.uint32), f16.cpu().numpy().view(np.uint32))
             net.close()
+
+
+def test_descriptor_demo_on_reference_sample_cloud(torch_gpu, net_dir, tmp_path):
+    """data/sample/3dmatch-demo/single-depth-2.ply (284 874 points, a
+    299 x 255 x 263 = 20.05 M-voxel grid, above 2^24, so core/demo.cu:32-34's
+    float32 decode mis-places voxels): descriptor_demo --decode reference
+    writes the reference's TDF bit for bit, the keypoints it was given and
+    their descriptors (core/demo.cu:69-135, 236-274)."""
+    d, w = net_dir
+    for f in ("3dmatch-net-test.json", "3dmatch-weights-snapshot-137000.marvin"):
+        shutil.copy(d / f, tmp_path / f)
+    ply = os.path.join(GOLDEN, "sample_clouds", "single-depth-2.ply")
+    cloud = tdm.read_ply_points(ply)
+    assert len(cloud) == 284_874
+    idx = synth.keypoint_indices(len(cloud), 6, seed=44)
+    (tmp_path / "kp.txt").write_text("\n".join(str(i) for i in idx))
+    run([os.path.join(BIN, "descriptor_demo"), ply, "test", "--keypoint-indices", "kp.txt",
+         "--tdf-bin", "tdf.bin", "--decode", "reference"], str(tmp_path))
+    vs = np.float32(0.01)
+    dims, origin = oracle.grid_params(cloud, vs, 15)
+    assert list(dims) == [299, 255, 263]
+    occ = oracle.occupancy(cloud, dims, origin, vs)
+    ref = oracle.compute_tdf(occ, vs, np.float32(0.05), "reference")
+    exact = oracle.compute_tdf(occ, vs, np.float32(0.05), "exact")
+    assert np.count_nonzero(ref != exact) > 1000  # the decodes differ on this grid
+    g, go, gvs, gtr = tdm.read_grid_bin(str(tmp_path / "tdf.bin"))
+    assert np.array_equal(go, origin) and gvs == vs and gtr == np.float32(0.05)
+    assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
+    # keypts.bin: float32 N, then N x 3 world coordinates
+    raw = open(tmp_path / "test.keypts.bin", "rb").read()
+    assert struct.unpack("<f", raw[:4])[0] == 6.0 and len(raw) == 4 + 6 * 12
+    kp = tdm.read_keypts_bin(str(tmp_path / "test.keypts.bin"))
+    assert np.array_equal(kp, cloud[idx])
+    # desc.3dmatch.bin: float32 N, float32 512, then N x 512
+    raw = open(tmp_path / "test.desc.3dmatch.bin", "rb").read()
+    assert struct.unpack("<2f", raw[:8]) == (6.0, 512.0) and len(raw) == 8 + 6 * 512 * 4
+    desc = tdm.read_desc_bin(str(tmp_path / "test.desc.3dmatch.bin"))
+    patches = oracle.extract_patches(ref, oracle.keypoint_grid(kp, origin, vs))
+    want = oracle.net_forward_fp64(patches, w, netdef.layer_shapes())[-1].reshape(6, 512)
+    assert rel_l2(desc, want) <= 1e-5
