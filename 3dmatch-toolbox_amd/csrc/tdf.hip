@@ -25,6 +25,12 @@
 
 #pragma clang fp contract(off)
 
+// Device memory of tdm_tdf_from_points_noalloc: the bit grid and its tile
+// flags (bits_bytes) of one call.  Each GPU holds its own copy for the life of
+// the process; host code reaches it only through the per-device address that
+// static_scratch_ptr looks up, never through this name.
+static __device__ __align__(256) unsigned char scratch[TDM_TDF_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int kMaxR = 8;          // largest supported search radius (voxels)
@@ -1073,6 +1079,28 @@ int check_points_args(const float *pts, int num_pts, const int dims[3], const fl
     return check_dims(dims[0], dims[1], dims[2]);
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[kMaxDevices];
+
+// The address of `scratch` on the current device, looked up once per device
+// (with that device current) and kept.
+int static_scratch_ptr(void **out) {
+    int dev = 0;
+    TDM_HIP(hipGetDevice(&dev));
+    TDM_CHECK(dev >= 0 && dev < kMaxDevices, TDM_ERR_UNSUPPORTED, "device %d beyond %d", dev,
+              kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[dev]) {
+        void *p = nullptr;
+        TDM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TDM_CHECK(p, TDM_ERR_HIP, "no device address for the static scratch");
+        g_static_ptr[dev] = p;
+    }
+    *out = g_static_ptr[dev];
+    return TDM_OK;
+}
+
 }  // namespace
 
 namespace tdm {
@@ -1107,9 +1135,9 @@ int tdm_grid_params(tdm_ctx *ctx, const float *pts, int num_pts, float voxel_siz
     TDM_TRY(tdm::resolve_ctx(ctx, &ctx));
     hipStream_t s = tdm::as_stream(stream);
     const int nblocks = (int)std::min<long>(((long)num_pts + 255) / 256, 1024);
-    void *scratch = nullptr;
-    TDM_TRY(tdm::ctx_scratch(ctx, 1, sizeof(float) * 6 * (nblocks + 1), &scratch, s));
-    float *part = static_cast<float *>(scratch);
+    void *area = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 1, sizeof(float) * 6 * (nblocks + 1), &area, s));
+    float *part = static_cast<float *>(area);
     float *res = part + 6 * nblocks;
     bbox_partial_kernel<<<nblocks, 256, 0, s>>>(pts, num_pts, part);
     TDM_LAUNCHED();
@@ -1188,6 +1216,37 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
     return points_to_tdf(p, pts, num_pts, dims, origin, voxel_size, r, decode, work, tdf, s);
 }
 
+int tdm_tdf_from_points_noalloc(const float *pts, int num_pts, const int dims[3],
+                                const float origin[3], float voxel_size, float trunc_margin,
+                                int decode, float *tdf, void *stream) {
+    tdm::clear_error();
+    TDM_TRY(check_points_args(pts, num_pts, dims, origin, tdf));
+    TDM_CHECK(decode == TDM_DECODE_EXACT || decode == TDM_DECODE_REFERENCE, TDM_ERR_INVALID_ARG,
+              "bad decode mode %d", decode);
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    // Below 2^24 voxels the float32 decode of core/demo.cu:32-34 is exact, so
+    // both modes are the same computation; above it the reference decode
+    // needs a whole-grid work buffer, which this entry point does not have.
+    TDM_CHECK(decode == TDM_DECODE_EXACT || nvox <= ((size_t)1 << 24), TDM_ERR_UNSUPPORTED,
+              "reference decode of a %zu-voxel grid needs tdm_tdf_from_points", nvox);
+    const size_t need = bits_bytes(dims[0], dims[1], dims[2]);
+    TDM_CHECK(need <= TDM_TDF_NOALLOC_SCRATCH_BYTES, TDM_ERR_UNSUPPORTED,
+              "grid %d x %d x %d needs %zu bytes of bit grid, above the %d-byte static scratch",
+              dims[0], dims[1], dims[2], need, TDM_TDF_NOALLOC_SCRATCH_BYTES);
+    int r = 0;
+    TDM_TRY(search_radius(voxel_size, trunc_margin, &r));
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return tdm::fail(TDM_ERR_NO_DEVICE, "no HIP device available");
+    }
+    (void)hipGetLastError();  // an earlier caller's error is not this call's
+    void *p = nullptr;
+    TDM_TRY(static_scratch_ptr(&p));
+    return points_to_tdf(p, pts, num_pts, dims, origin, voxel_size, r, TDM_DECODE_EXACT, tdf, tdf,
+                         tdm::as_stream(stream));
+}
+
 int tdm_keypoints_to_grid(const float *kp, int num, const float origin[3], float voxel_size,
                           int *grid_out) {
     tdm::clear_error();
@@ -1244,9 +1303,9 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     const int margin = reach * kAT;
     const long nb = (nt + 1023) / 1024;  // scan blocks
     const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1 + nb) + 16 + sizeof(float4) * (size_t)num_pts;
-    void *scratch = nullptr;
-    TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &scratch, s));
-    int *counts = static_cast<int *>(scratch);
+    void *area = nullptr;
+    TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &area, s));
+    int *counts = static_cast<int *>(area);
     int *offsets = counts + nt;  // nt + 1 entries
     int *bsum = offsets + nt + 1;  // nb entries
     float4 *binned = reinterpret_cast<float4 *>(

@@ -216,6 +216,19 @@ def tdf_from_points(pts, dims, origin, voxel_size, trunc, tdf, decode=DECODE_EXA
                                      _p(tdf), _stream(stream)))
 
 
+def tdf_from_points_noalloc(pts, dims, origin, voxel_size, trunc, tdf, decode=DECODE_EXACT,
+                            stream=None):
+    """tdm_tdf_from_points_noalloc: the same TDF with the bit grid in the
+    library's static per-GPU scratch (no allocation; calls on one GPU must not
+    overlap)."""
+    _check(lib().tdm_tdf_from_points_noalloc(_p(pts), c_int(pts.shape[0]), _int3(dims),
+                                             _float3(origin), _f(voxel_size), _f(trunc),
+                                             c_int(decode), _p(tdf), _stream(stream)))
+
+
+NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024
+
+
 def tdf_accurate(pts, dims, origin, voxel_size, trunc, tdf, stream=None) -> None:
     """Voxel-centre to nearest-point TDF (training/match.hpp:13-51)."""
     _check(lib().tdm_tdf_accurate(None, _p(pts), c_int(pts.shape[0]), _int3(dims),

@@ -191,6 +191,22 @@ int tdm_tdf_from_points(tdm_ctx *ctx, const float *pts, int num_pts, const int d
                         const float origin[3], float voxel_size, float trunc_margin, int decode,
                         float *tdf, void *stream);
 
+/* No counterpart in the reference.  tdm_tdf_from_points for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or
+ * allocating routine and takes no context.  Its bit grid and tile flags live
+ * in a static device array of TDM_TDF_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends); a grid needing more (about 16 bytes per 128
+ * voxels) fails with TDM_ERR_UNSUPPORTED, as does reference decode above
+ * 2^24 voxels (below it both decodes are the same computation).  Every call
+ * on one GPU shares that array: calls that overlap on a GPU (other streams,
+ * other threads) race and must be ordered by the caller.  Results are
+ * bit-identical to tdm_tdf_from_points. */
+#define TDM_TDF_NOALLOC_SCRATCH_BYTES (16 * 1024 * 1024)
+int tdm_tdf_from_points_noalloc(const float *pts, int num_pts, const int dims[3],
+                                const float origin[3], float voxel_size, float trunc_margin,
+                                int decode, float *tdf, void *stream);
+
 /* Accurate TDF (training/match.hpp:13-51 Get1NNSearchDist,
  * deprecated/pointCloud2AccTDF.m): tdf = 1 - min(d, trunc_margin) /
  * trunc_margin with d the Euclidean distance from each voxel centre
