@@ -329,6 +329,18 @@ def estimate_rt(x1, x2, rt, stream=None) -> None:
                                  _stream(stream)))
 
 
+def alignment_ratio(pts1, pts2, rt, radius=0.05, stream=None):
+    """ratioAligned of register2Fragments.m:90-97 on the device: (fraction of
+    pts1 within radius of pts2 moved by rt, fraction of moved pts2 within
+    radius of pts1).  pts1, pts2: device [n, 3] float; rt: host 3x4 float64."""
+    r = np.ascontiguousarray(np.asarray(rt, np.float64).reshape(12))
+    out = (ctypes.c_double * 2)()
+    _check(lib().tdm_alignment_ratio(None, _p(pts1), c_int(pts1.shape[0]), _p(pts2),
+                                     c_int(pts2.shape[0]), r.ctypes.data_as(c_void_p),
+                                     _f(radius), out, _stream(stream)))
+    return out[0], out[1]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget", c_size_t), ("max_inflight", c_int), ("keypoints", c_int),
                 ("max_pass_keypoints", c_int), ("voxel_size", c_float),

@@ -407,6 +407,18 @@ int tdm_ransac_rt(tdm_ctx *ctx, const float *kp1, const float *kp2, const int *p
 int tdm_estimate_rt(tdm_ctx *ctx, const float *x1, const float *x2, int num, double *rt,
                     void *stream);
 
+/* Alignment ratios of evaluation/geometric-registration/register2Fragments.m
+ * :68-69, 90-97: fragment 2 moved by rt (row-major 3x4, x1 = R x2 + t,
+ * applied in float with R and t rounded to float, as MATLAB evaluates the
+ * double matrix times the single cloud), then ratio[0] = the fraction of
+ * fragment-1 points with a moved fragment-2 point closer than radius (0.05
+ * there), ratio[1] = the fraction of moved fragment-2 points with a
+ * fragment-1 point closer than radius.  Distances are sqrtf((dx*dx + dy*dy)
+ * + dz*dz) in float.  pts1, pts2: device xyz floats; ratio: host.
+ * Synchronizes `stream`. */
+int tdm_alignment_ratio(tdm_ctx *ctx, const float *pts1, int num1, const float *pts2, int num2,
+                        const double rt[12], float radius, double ratio[2], void *stream);
+
 /* ---- registration serving ---------------------------------------------------------
  *
  * A request registers fragment 2 onto fragment 1, the pipeline of

@@ -18,6 +18,11 @@ Reference behaviour restated here:
   (p = 0.99, `>=` update, N = max(10, log(1-p)/log(pNoOutliers)), stop when
   trialcount > maxTrials).  The trial samples come from the counter-based
   generator below (MATLAB's RandStream cannot be reproduced).
+- ratioAligned: evaluation/geometric-registration/register2Fragments.m:68-69,
+  90-97 (fragment 2 moved by the estimate in single precision, then the
+  fraction of each fragment's points with a point of the other closer than
+  0.05).  MATLAB's sgemm order and its KD-tree's distance arithmetic are not
+  reproducible; the restatement fixes a left-to-right float32 order.
 """
 from __future__ import annotations
 
@@ -262,3 +267,63 @@ def random_rotation(rng: np.random.Generator, max_angle=None) -> np.ndarray:
     a = rng.uniform(0, max_angle)
     K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
     return np.eye(3) + math.sin(a) * K + (1 - math.cos(a)) * K @ K
+
+
+def move_points(pts: np.ndarray, rt: np.ndarray) -> np.ndarray:
+    """Fragment 2 moved by [R|t] in float32 (register2Fragments.m:68: MATLAB
+    multiplies the double matrix by the single cloud in single), each
+    coordinate ((r0*x + r1*y) + r2*z) + t without contraction."""
+    R = np.asarray(rt, np.float64).reshape(3, 4)[:, :3].astype(np.float32)
+    t = np.asarray(rt, np.float64).reshape(3, 4)[:, 3].astype(np.float32)
+    p = np.asarray(pts, np.float32)
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    return np.stack([((R[i, 0] * x + R[i, 1] * y) + R[i, 2] * z) + t[i] for i in range(3)], 1)
+
+
+def fraction_within(queries: np.ndarray, targets: np.ndarray, radius: float) -> float:
+    """Fraction of queries whose nearest target is closer than radius, the
+    distance being sqrtf((dx*dx + dy*dy) + dz*dz) in float32
+    (register2Fragments.m:91-96 with multiQueryKNNSearchImpl's 1-NN).  A
+    float64 KD-tree (scipy cKDTree) settles the clear cases; queries whose
+    nearest float64 distance lies within 1e-5 of the radius are decided by the
+    float32 distance to every target in reach."""
+    from scipy.spatial import cKDTree
+
+    q = np.asarray(queries, np.float32)
+    t = np.asarray(targets, np.float32)
+    r32 = np.float32(radius)
+    tree = cKDTree(t.astype(np.float64))
+    d, _ = tree.query(q.astype(np.float64), k=1, workers=-1)
+    slack = 1e-5 * max(1.0, float(radius))
+    hit = d < float(r32) - slack
+    unsure = np.nonzero(np.abs(d - float(r32)) <= slack)[0]
+    if len(unsure):
+        cands = tree.query_ball_point(q[unsure].astype(np.float64), r=float(r32) + 2 * slack,
+                                       workers=-1)
+        for k, i in enumerate(unsure):
+            c = np.asarray(cands[k], dtype=np.int64)
+            if len(c) == 0:
+                continue
+            dd = q[i][None, :] - t[c]
+            d2 = (dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2]
+            hit[i] = bool(np.any(np.sqrt(d2) < r32))
+    return float(np.count_nonzero(hit)) / len(q)
+
+
+def fraction_within_brute(queries, targets, radius) -> float:
+    """The same by brute force over every pair in float32 (small inputs)."""
+    q = np.asarray(queries, np.float32)
+    t = np.asarray(targets, np.float32)
+    hits = 0
+    for i in range(len(q)):
+        dd = q[i][None, :] - t
+        d2 = (dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2]
+        hits += bool(np.any(np.sqrt(d2) < np.float32(radius)))
+    return hits / len(q)
+
+
+def alignment_ratio(pts1, pts2, rt, radius=0.05):
+    """ratioAligned of register2Fragments.m:90-97."""
+    moved = move_points(pts2, rt)
+    return fraction_within(pts1, moved, radius), fraction_within(moved, pts1, radius)
+

@@ -186,14 +186,15 @@ def test_descriptor_demo_on_reference_sample_cloud(torch_gpu, net_dir, tmp_path)
     run([os.path.join(BIN, "descriptor_demo"), ply, "test", "--keypoint-indices", "kp.txt",
          "--tdf-bin", "tdf.bin", "--decode", "reference"], str(tmp_path))
     vs = np.float32(0.01)
+    trunc = vs * np.float32(5)  # core/demo.cu: trunc_margin = voxel_size * 5 in float
     dims, origin = oracle.grid_params(cloud, vs, 15)
     assert list(dims) == [299, 255, 263]
     occ = oracle.occupancy(cloud, dims, origin, vs)
-    ref = oracle.compute_tdf(occ, vs, np.float32(0.05), "reference")
-    exact = oracle.compute_tdf(occ, vs, np.float32(0.05), "exact")
+    ref = oracle.compute_tdf(occ, vs, trunc, "reference")
+    exact = oracle.compute_tdf(occ, vs, trunc, "exact")
     assert np.count_nonzero(ref != exact) > 1000  # the decodes differ on this grid
     g, go, gvs, gtr = tdm.read_grid_bin(str(tmp_path / "tdf.bin"))
-    assert np.array_equal(go, origin) and gvs == vs and gtr == np.float32(0.05)
+    assert np.array_equal(go, origin) and gvs == vs and gtr == trunc
     assert np.array_equal(g.view(np.uint32), ref.view(np.uint32))
     # keypts.bin: float32 N, then N x 3 world coordinates
     raw = open(tmp_path / "test.keypts.bin", "rb").read()

@@ -8,6 +8,7 @@ import pytest
 
 import registration_oracle as ro
 import tdm
+from tdm import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -213,3 +214,53 @@ def test_nn_common_offset_and_dense_clusters(torch_gpu):
     B[650:] = B[600] + rng.normal(scale=1e-4, size=(50, 512)).astype(np.float32)
     _, _, refined = _check_nn(torch_gpu, A, B)
     print("refined queries", refined, flush=True)
+
+
+# ---- ratioAligned (register2Fragments.m:90-97) ---------------------------------------
+
+
+def _ratio(torch, p1, p2, rt, radius=0.05):
+    return tdm.alignment_ratio(_dev(torch, p1, np.float32), _dev(torch, p2, np.float32), rt, radius)
+
+
+def test_alignment_ratio_fragment_pair_vs_oracle(torch_gpu):
+    """A fragment pair of the serving workload (300k + 250k points) under its
+    true motion and under a perturbed one: both ratios equal the oracle's
+    (counts are exact, so the fractions are bit-identical)."""
+    p1, p2, R, t = synth.fragment_pair(77, n1=300_000, n2=250_000)
+    true_rt = np.hstack([R, t[:, None]])
+    rng = np.random.default_rng(5)
+    off_rt = np.hstack([ro.random_rotation(rng, max_angle=0.05) @ R, (t + 0.03)[:, None]])
+    for rt in (true_rt, off_rt):
+        got = _ratio(torch_gpu, p1, p2, rt)
+        want = ro.alignment_ratio(p1, p2, rt)
+        print("ratios", got, want, flush=True)
+        assert got == want
+    assert _ratio(torch_gpu, p1, p2, true_rt)[0] > 0.3
+
+
+def test_alignment_ratio_boundaries_and_wide_boxes(torch_gpu):
+    rng = np.random.default_rng(9)
+    eye = np.hstack([np.eye(3), np.zeros((3, 1))])
+    # pairs exactly at the radius (not closer) and one float below it
+    q = np.zeros((2, 3), np.float32)
+    q[1, 1] = 3.0
+    t = np.array([[0.05, 0, 0], [0, 3.0, 0.05]], np.float32)
+    assert _ratio(torch_gpu, q, t, eye) == ro.alignment_ratio(q, t, eye) == (0.0, 0.0)
+    t[0, 0] = np.nextafter(np.float32(0.05), np.float32(0))
+    assert _ratio(torch_gpu, q, t, eye) == ro.alignment_ratio(q, t, eye) == (0.5, 0.5)
+    # far outliers stretch the boxes (wider cells), a thin slab, clustered points
+    a = rng.random((20_000, 3), dtype=np.float32)
+    a[:5] *= np.float32(5000.0)
+    b = a[rng.permutation(len(a))[:15_000]] + rng.normal(0, 0.03, (15_000, 3)).astype(np.float32)
+    slab = np.stack([rng.random(30_000) * 800, rng.random(30_000) * 0.1, rng.random(30_000) * 0.1],
+                    1).astype(np.float32)
+    slab2 = slab + rng.normal(0, 0.04, slab.shape).astype(np.float32)
+    clus = (rng.normal(0, 0.01, (10_000, 3))).astype(np.float32)
+    for p1, p2 in ((a, b), (slab, slab2), (clus, clus[::-1].copy())):
+        R = ro.random_rotation(rng, max_angle=0.02)
+        rt = np.hstack([R, rng.normal(0, 0.01, (3, 1))])
+        got = _ratio(torch_gpu, p1, p2, rt)
+        want = ro.alignment_ratio(p1, p2, rt)
+        print("case", len(p1), len(p2), got, want, flush=True)
+        assert got == want

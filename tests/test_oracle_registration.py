@@ -165,3 +165,40 @@ def test_ransac_small_sets():
     rt, inl, best, run, _ = ro.ransac_fit_rt(x, x)
     assert list(inl) == [0, 1, 2] and run == 0
     assert np.abs(rt - np.hstack([np.eye(3), np.zeros((3, 1))])).max() < 1e-12
+
+
+# ---- ratioAligned (register2Fragments.m:90-97) ---------------------------------------
+
+
+def test_alignment_fraction_pinned_by_brute_force_and_closed_forms():
+    rng = np.random.default_rng(71)
+    a = rng.random((300, 3), dtype=np.float32)
+    b = rng.random((250, 3), dtype=np.float32) * np.float32(1.2)
+    for r in (0.02, 0.05, 0.2):
+        assert ro.fraction_within(a, b, r) == ro.fraction_within_brute(a, b, r)
+    # identity transform of the same cloud: every point finds itself
+    eye = np.hstack([np.eye(3), np.zeros((3, 1))])
+    assert ro.alignment_ratio(a, a, eye) == (1.0, 1.0)
+    # a copy shifted by more than the radius: nothing aligns
+    far = np.hstack([np.eye(3), [[10.0], [0.0], [0.0]]])
+    assert ro.alignment_ratio(a, a, far) == (0.0, 0.0)
+    # exactly at the radius is not closer than it (sqrtf(fl(x*x)) == |x|)
+    q = np.array([[0, 0, 0]], np.float32)
+    t = np.array([[0.05, 0, 0]], np.float32)
+    assert ro.fraction_within(q, t, 0.05) == 0.0
+    t2 = np.array([[np.nextafter(np.float32(0.05), np.float32(0))], [0], [0]], np.float32).T
+    assert ro.fraction_within(q, t2, 0.05) == 1.0
+    # a planted mistake (<= instead of <) is caught by that boundary
+    dd = q[0] - t
+    d2 = (dd[:, 0] * dd[:, 0] + dd[:, 1] * dd[:, 1]) + dd[:, 2] * dd[:, 2]
+    assert np.sqrt(d2)[0] <= np.float32(0.05) and not np.sqrt(d2)[0] < np.float32(0.05)
+
+
+def test_move_points_matches_float64_transform():
+    rng = np.random.default_rng(72)
+    p = rng.random((1000, 3), dtype=np.float32) * 4 - 2
+    R = ro.random_rotation(rng, max_angle=np.pi / 3)
+    rt = np.hstack([R, rng.random((3, 1)) - 0.5])
+    got = ro.move_points(p, rt)
+    want = p.astype(np.float64) @ rt[:, :3].T + rt[:, 3]
+    assert np.abs(got - want).max() < 1e-5
