@@ -450,13 +450,27 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
 // fetch each A row for more columns.  Every variant feeds each output the
 // same k in the same order (lane group g reads bytes 16g + 64h of a stage),
 // so all variants give bit-identical results and can be picked by timing.
-// A stage is 128 or 64 bytes of k per row; LDS rows are padded by 16 bytes.
-// Both stage sizes walk k in the same order (64-byte halves in sequence).
+// A stage is 128 or 64 bytes of k per row.  Both stage sizes walk k in the
+// same order (64-byte halves in sequence).
+//
+// LDS rows are unpadded and the 16-byte pieces of a row are XOR-swizzled:
+// piece c of row r sits at piece c ^ swz(r).  A fragment read has lane l take
+// piece g (+ 4h) of row l % 16; a staging store has consecutive threads take
+// consecutive pieces of consecutive rows.  With the swizzle, the 16 lanes of
+// a read land on 16 different 16-byte bank slots of a 256-byte bank line, and
+// any 8 consecutive lanes on 8 different slots of a 128-byte line, for every
+// g, h and block; with the earlier 16-byte row padding (80-byte pitch) the
+// counters showed ~5 cycles of bank conflict per LDS instruction.
+template <int STAGE>
+__device__ inline int swz(int r) {
+    return STAGE == 64 ? ((r >> 1) ^ (r >> 3)) & 3 : ((r >> 1) ^ (r << 2)) & 7;
+}
+
 template <int WMW, int WTM, int WTN, int STAGE = 128>
 struct Tile {
     static constexpr int WNW = 4 / WMW;
     static constexpr int BM = WMW * WTM, BN = WNW * WTN;
-    static constexpr int ROW = STAGE + 16;
+    static constexpr int ROW = STAGE;
     static constexpr int LDS = 2 * (BM + BN) * ROW;
     static constexpr int OCC = LDS <= 80 * 1024 ? 2 : 1;  // workgroups per CU
 };
@@ -492,7 +506,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         const int idx = tid + 256 * j, row = idx / PCS, pc = idx % PCS;
         const long m = min(mb + row, p.M - 1);
         a_src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + pc * 16;
-        a_dst[j] = row * kLdsRow + pc * 16;
+        a_dst[j] = row * kLdsRow + ((pc ^ swz<STAGE>(row)) << 4);
     }
     const unsigned char *b_src[B_PT];
     int b_dst[B_PT];
@@ -500,7 +514,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
     for (int j = 0; j < B_PT; ++j) {
         const int idx = tid + 256 * j, col = idx / PCS, pc = idx % PCS;
         b_src[j] = w + ((long)(nb + col) * K) * (long)sizeof(T) + pc * 16;
-        b_dst[j] = (BM + col) * kLdsRow + pc * 16;
+        b_dst[j] = (BM + col) * kLdsRow + ((pc ^ swz<STAGE>(BM + col)) << 4);
     }
 
     const int cps = C / SK;  // stages per tap
@@ -514,13 +528,15 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         s0 = (int)((long)nc * tz / p.splits) * CG;
         s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
     }
+    // Stages are fetched strictly in order s0, s0+1, ...: the tap and channel
+    // offset advance with each fetch instead of being divided out of s.
+    int ftap = s0 / cps, fc0 = (s0 - ftap * cps) * SK;
     // Two register sets: the loads of stage s+2 are issued before the MFMAs
     // of stage s, and stage s+1 (loaded one stage earlier) goes to LDS after
     // them, so a global load has two stages of compute to arrive.
     uint4 ra0[A_PT], rb0[B_PT], ra1[A_PT], rb1[B_PT];
-    auto fetch = [&](int s, uint4 (&ra)[A_PT], uint4 (&rb)[B_PT]) {
-        const int tap = s / cps;
-        const int c0 = (s - tap * cps) * SK;
+    auto fetch = [&](uint4 (&ra)[A_PT], uint4 (&rb)[B_PT]) {
+        const int tap = ftap, c0 = fc0;
         int kz = 0, ky = 0, kx = 0;
         if (ks == 3) {
             kz = tap / 9;
@@ -533,6 +549,11 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         for (int j = 0; j < A_PT; ++j) ra[j] = ld16(a_src[j] + aoff);
 #pragma unroll
         for (int j = 0; j < B_PT; ++j) rb[j] = ld16(b_src[j] + boff);
+        fc0 += SK;
+        if (fc0 == C) {
+            fc0 = 0;
+            ++ftap;
+        }
     };
     auto stash = [&](int buf, const uint4 (&ra)[A_PT], const uint4 (&rb)[B_PT]) {
 #pragma unroll
@@ -548,18 +569,31 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const int r16 = lane & 15, g = lane >> 4;
+    // Per-lane byte offsets of the fragment pieces within a stage buffer.
+    int a_off[IB][HALVES], b_off[JB][HALVES];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+        const int row = wm * WTM + 16 * i + r16;
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h)
+            a_off[i][h] = row * kLdsRow + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int row = BM + wn * WTN + 16 * j + r16;
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h)
+            b_off[j][h] = row * kLdsRow + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
+    }
     auto compute = [&](int buf) {
-        const unsigned char *A = &lds[buf][(wm * WTM + r16) * kLdsRow + g * 16];
-        const unsigned char *B = &lds[buf][(BM + wn * WTN + r16) * kLdsRow + g * 16];
+        const unsigned char *L = lds[buf];
 #pragma unroll
         for (int h = 0; h < HALVES; ++h) {
             uint4 a[IB], b[JB];
 #pragma unroll
-            for (int i = 0; i < IB; ++i)
-                a[i] = *reinterpret_cast<const uint4 *>(A + 16 * i * kLdsRow + h * 64);
+            for (int i = 0; i < IB; ++i) a[i] = *reinterpret_cast<const uint4 *>(L + a_off[i][h]);
 #pragma unroll
-            for (int j = 0; j < JB; ++j)
-                b[j] = *reinterpret_cast<const uint4 *>(B + 16 * j * kLdsRow + h * 64);
+            for (int j = 0; j < JB; ++j) b[j] = *reinterpret_cast<const uint4 *>(L + b_off[j][h]);
             Mfma<T>::chunk(acc, a, b);
         }
     };
@@ -567,15 +601,15 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
     auto step = [&](int s, const uint4 (&ha)[A_PT], const uint4 (&hb)[B_PT], uint4 (&na)[A_PT],
                     uint4 (&nb)[B_PT]) {
         const int buf = (s - s0) & 1;
-        if (s + 2 < s1) fetch(s + 2, na, nb);
+        if (s + 2 < s1) fetch(na, nb);
         compute(buf);
         if (s + 1 < s1) stash(buf ^ 1, ha, hb);
         __syncthreads();
     };
     if (s0 < s1) {
-        fetch(s0, ra0, rb0);
+        fetch(ra0, rb0);
         stash(0, ra0, rb0);
-        if (s0 + 1 < s1) fetch(s0 + 1, ra1, rb1);
+        if (s0 + 1 < s1) fetch(ra1, rb1);
         __syncthreads();
         for (int s = s0; s < s1; s += 2) {
             step(s, ra1, rb1, ra0, rb0);
