@@ -868,16 +868,20 @@ __global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
         if (slot[k] >= 0) binned[T.cnt[slot[k]] + rank[k]] = p[k];
 }
 
-// One 8x8x8 tile per workgroup of 512 threads.  Points of the tiles within
-// `reach` tiles (reach*8 voxels >= the truncation radius plus the half voxel
-// a point may sit off its voxel) are staged through LDS 512 at a time, and
-// only those that can be within the truncation distance of one of the
-// tile's voxel centres are kept: the result is 1 - min(d, trunc)/trunc of the
-// nearest point, so a point at distance >= trunc from every centre changes
-// nothing.  The test uses the box of the centres, computed with the same
-// float operations as the centres (so it contains them); float rounding is
-// monotone, so every centre's d^2 is at least the box's, and the threshold
-// trunc^2 (1 + 2^-20) clears the rounding of trunc^2 itself.
+// One 8x8x8 tile per workgroup of 512 threads.  The grid is zeroed first (a
+// full-line fill), and a voxel's value is 0 whenever no point lies within
+// the truncation distance, so only voxels with a point in reach are stored.
+// The candidates are the points of the tiles within `reach` tiles (reach*8
+// voxels >= the truncation radius plus the half voxel a point may sit off
+// its voxel): the workgroup reads the (2 reach + 1)^3 bin ranges at once,
+// numbers their points consecutively (prefix sums in LDS), and streams them
+// 512 at a time through LDS, keeping only those that can be within the
+// truncation distance of one of the tile's voxel centres.  The test uses the
+// box of the centres, computed with the same float operations as the centres
+// (so it contains them); float rounding is monotone, so every centre's d^2 is
+// at least the box's, and the threshold trunc^2 (1 + 2^-20) clears the
+// rounding of trunc^2 itself.  A tile with no candidate returns at once.
+constexpr int kAccMaxBins = 9 * 9 * 9;  // (2 reach + 1)^3 with reach <= 4
 __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
                                                       const int *__restrict__ offsets, int dx,
                                                       int dy, int dz, int tx, int ty, int tz,
@@ -885,12 +889,44 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
                                                       float vs, float trunc,
                                                       float *__restrict__ tdf) {
     __shared__ float4 sp[512];
+    __shared__ int bstart[kAccMaxBins];
+    __shared__ int bpre[kAccMaxBins + 1];
     __shared__ int wave_n[8];
     const int t = blockIdx.x;
     const int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
     const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
     const int x = bx * kAT + lx, y = by * kAT + ly, z = bz * kAT + lz;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = max(0, bx - reach), y0 = max(0, by - reach), z0 = max(0, bz - reach);
+    const int nx = min(tx - 1, bx + reach) - x0 + 1, ny = min(ty - 1, by + reach) - y0 + 1,
+              nz = min(tz - 1, bz + reach) - z0 + 1;
+    const int nbins = nx * ny * nz;
+    // bin ranges, then their exclusive prefix (wave 0, 64 bins per step)
+    for (int k = threadIdx.x; k < nbins; k += 512) {
+        const int kx = k % nx, ky = (k / nx) % ny, kz = k / (nx * ny);
+        const long nt = ((long)(z0 + kz) * ty + (y0 + ky)) * tx + (x0 + kx);
+        const int a = offsets[nt];
+        bstart[k] = a;
+        bpre[k + 1] = offsets[nt + 1] - a;  // count, scanned below
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int run = 0;
+        for (int k0 = 0; k0 < nbins; k0 += 64) {
+            const int k = k0 + lane;
+            int v = k < nbins ? bpre[k + 1] : 0;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(v, off);
+                if (lane >= off) v += u;
+            }
+            if (k < nbins) bpre[k + 1] = run + v;
+            run += __shfl(v, 63);
+        }
+        if (lane == 0) bpre[0] = 0;
+    }
+    __syncthreads();
+    const int total = bpre[nbins];
+    if (total == 0) return;  // uniform: nothing within reach, the grid holds 0
     // voxel centre (origin is the centre of voxel 0)
     const float cx = ox + (float)x * vs, cy = oy + (float)y * vs, cz = oz + (float)z * vs;
     const float lox = ox + (float)(bx * kAT) * vs, hix = ox + (float)(bx * kAT + kAT - 1) * vs;
@@ -898,45 +934,47 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
     const float loz = oz + (float)(bz * kAT) * vs, hiz = oz + (float)(bz * kAT + kAT - 1) * vs;
     const float lim = trunc * trunc * (1.0f + 0x1p-20f);
     float best = __builtin_huge_valf();
-    for (int nz = max(0, bz - reach); nz <= min(tz - 1, bz + reach); ++nz)
-        for (int ny = max(0, by - reach); ny <= min(ty - 1, by + reach); ++ny)
-            for (int nx = max(0, bx - reach); nx <= min(tx - 1, bx + reach); ++nx) {
-                const long nt = ((long)nz * ty + ny) * tx + nx;
-                const int p0 = offsets[nt], p1 = offsets[nt + 1];
-                for (int c0 = p0; c0 < p1; c0 += 512) {
-                    const int cn = min(512, p1 - c0);
-                    bool keep = false;
-                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if ((int)threadIdx.x < cn) {
-                        q = binned[c0 + threadIdx.x];
-                        const float ex = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.0f);
-                        const float ey = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.0f);
-                        const float ez = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.0f);
-                        keep = ex * ex + ey * ey + ez * ez <= lim;
-                    }
-                    const unsigned long long m = __ballot(keep);
-                    __syncthreads();  // the previous chunk's reads of sp are done
-                    if (lane == 0) wave_n[wave] = __popcll(m);
-                    __syncthreads();
-                    int pos = __popcll(m & ((1ull << lane) - 1ull)), kept = 0;
-                    for (int w = 0; w < 8; ++w) {
-                        const int c = wave_n[w];
-                        pos += w < wave ? c : 0;
-                        kept += c;
-                    }
-                    if (keep) sp[pos] = q;
-                    __syncthreads();
-                    for (int k = 0; k < kept; ++k) {
-                        const float4 q = sp[k];
-                        const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
-                        const float d2 = ex * ex + ey * ey + ez * ez;
-                        best = fminf(best, d2);
-                    }
-                }
+    for (int c0 = 0; c0 < total; c0 += 512) {
+        const int c = c0 + (int)threadIdx.x;
+        bool keep = false;
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (c < total) {
+            // the bin holding candidate c: last k with bpre[k] <= c
+            int lo = 0, hi = nbins - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (bpre[mid] <= c) lo = mid;
+                else hi = mid - 1;
             }
+            q = binned[bstart[lo] + (c - bpre[lo])];
+            const float ex = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.0f);
+            const float ey = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.0f);
+            const float ez = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.0f);
+            keep = ex * ex + ey * ey + ez * ez <= lim;
+        }
+        const unsigned long long m = __ballot(keep);
+        __syncthreads();  // the previous chunk's reads of sp are done
+        if (lane == 0) wave_n[wave] = __popcll(m);
+        __syncthreads();
+        int pos = __popcll(m & ((1ull << lane) - 1ull)), kept = 0;
+        for (int w = 0; w < 8; ++w) {
+            const int cnt = wave_n[w];
+            pos += w < wave ? cnt : 0;
+            kept += cnt;
+        }
+        if (keep) sp[pos] = q;
+        __syncthreads();
+        for (int k = 0; k < kept; ++k) {
+            const float4 p = sp[k];
+            const float ex = cx - p.x, ey = cy - p.y, ez = cz - p.z;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            best = fminf(best, d2);
+        }
+    }
     if (x >= dx || y >= dy || z >= dz) return;
     const float d = sqrtf(best);  // sqrt is monotone: the minimum of the distances
-    tdf[((long)z * dy + y) * dx + x] = 1.0f - fminf(d, trunc) / trunc;
+    const float v = 1.0f - fminf(d, trunc) / trunc;
+    if (v != 0.0f) tdf[((long)z * dy + y) * dx + x] = v;  // 0 is already there
 }
 
 int check_dims(int dx, int dy, int dz) {
@@ -1311,7 +1349,9 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     float4 *binned = reinterpret_cast<float4 *>(
         (reinterpret_cast<uintptr_t>(bsum + nb) + 15) & ~uintptr_t(15));
     const unsigned fb = (unsigned)std::min<long>((nt + 255) / 256, 16384);
-    fill_u32_kernel<<<fb, 256, 0, s>>>(reinterpret_cast<unsigned *>(counts), (size_t)nt, 0u);
+    // one launch zeroes the output grid (full lines) and the bin counts
+    clear_regions(tdf, (size_t)dims[0] * dims[1] * dims[2], reinterpret_cast<unsigned char *>(counts),
+                  sizeof(int) * (size_t)nt, s);
     TDM_LAUNCHED();
     if (num_pts > 0) {
         acc_count_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),

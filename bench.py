@@ -135,6 +135,38 @@ def calibrate_fp8(torch, net, js, wt, pts, keypts):
     return scales
 
 
+def net_passes(torch, precision, batch, reps=3):
+    """Descriptor forward only: `reps` passes of `batch` keypoints of the
+    200k-point fragment's TDF through the autotuned net (for kernel traces and
+    counters of the convolutions); per-layer ms per pass by HIP events."""
+    import numpy as np
+
+    import tdm
+    from tdm import netdef
+
+    cloud, keypts, weights = descriptor_workload(200_000, batch)
+    tmp = tempfile.mkdtemp(prefix="tdm_net_")
+    js, wt = os.path.join(tmp, "net.json"), os.path.join(tmp, "w.marvin")
+    netdef.write_net_json(js)
+    tdm.marvin_write(wt, weights)
+    prec = {"fp32": tdm.PREC_FP32, "bf16": tdm.PREC_BF16, "fp16": tdm.PREC_FP16}[precision]
+    pipe = DescriptorPipeline(torch, cloud, keypts, js, wt, prec, batch)
+    pipe.step()  # TDF, keypoints and one warm pass
+    torch.cuda.synchronize()
+    pipe.net.set_profiling(True)
+    for _ in range(reps):
+        pipe.net.forward_grid(pipe.tdf, pipe.dims, pipe.kp_dev, pipe.feat)
+    torch.cuda.synchronize()
+    layers = {}
+    for i, name in enumerate(pipe.net.layer_names):
+        ms, passes, _ = pipe.net.layer_stats(i)
+        layers[name] = ms / max(passes, 1)
+    out = {"precision": precision, "batch": batch, "tile_variants": pipe.variants,
+           "ms_per_pass": layers}
+    pipe.net.close()
+    return out
+
+
 def time_steps(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -363,11 +395,23 @@ def mutual_nn(torch, a, b):
     return ab, mutual
 
 
-def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8):
-    """Precision form: fragments = views 0..n_frag-1 of the synthetic scene,
-    n_kp keypoints each; bf16 descriptors vs the fp32 path (rel-L2), mutual-NN
-    agreement on the pair (0, 1), and per-layer rel-L2 of both paths vs the
-    FP64 oracle on a few patches."""
+def _nn_margins(torch, a, b):
+    """Per row of a: nearest and second-nearest Euclidean distances to b's rows
+    (fp64 on the device, exact enough to rank the fp32 descriptors)."""
+    a64, b64 = a.double(), b.double()
+    d2 = (a64 * a64).sum(1, keepdim=True) + (b64 * b64).sum(1)[None, :] - 2.0 * a64 @ b64.T
+    top = torch.topk(d2, 2, dim=1, largest=False).values.clamp_min(0).sqrt()
+    return top[:, 0], top[:, 1]
+
+
+def precision_study(torch, js, wt, weights, n_frag=20, n_kp=5000, layer_patches=8):
+    """Precision form (20 fragments x 5000 keypoints = 100k patches): for each
+    precision, descriptor rel-L2 vs the fp32 path and mutual-NN agreement with
+    fp32 on the fragment pairs (0,1), (2,3), ..., (18,19); the fp32 floor (the
+    same agreement between two fp32 nets whose split-K summation orders
+    differ); the NN-margin distribution of pair (0,1) with each precision's
+    disagreement per margin bucket; per-layer rel-L2 vs the FP64 oracle on a
+    few patches."""
     import numpy as np
 
     import tdm
@@ -377,11 +421,13 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
     import oracle
 
     dev = torch.device("cuda:0")
-    nets = {p: tdm.Net(js, p, max_batch=1024)
-            for p in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8)}
+    names = {tdm.PREC_FP32: "fp32", tdm.PREC_BF16: "bf16", tdm.PREC_FP16: "fp16",
+             tdm.PREC_FP8: "fp8"}
+    nets = {p: tdm.Net(js, p, max_batch=1024) for p in names}
+    # fp32 with another summation order: split-K counts follow max_batch
+    nets["fp32_alt"] = tdm.Net(js, tdm.PREC_FP32, max_batch=64)
     for n in nets.values():
         n.load_weights(wt)
-    # fp8 activation scales from the fp32 net on 256 patches of fragment 0
     c0 = synth.fragment(200_000, view=0, seed=10)
     calibrate_fp8(torch, nets[tdm.PREC_FP8], js, wt, torch.from_numpy(c0).to(dev),
                   c0[synth.keypoint_indices(len(c0), 256, seed=999)])
@@ -404,26 +450,49 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
             patches = torch.empty(layer_patches, 30, 30, 30, device=dev)
             tdm.extract_patches(tdf, dims, kgd[:layer_patches], patches)
             first_patches = patches
+        del tdf
     torch.cuda.synchronize()
     d32 = torch.cat(desc[tdm.PREC_FP32])
-    ab32, m32 = mutual_nn(torch, desc[tdm.PREC_FP32][0], desc[tdm.PREC_FP32][1])
+    pairs = [(2 * k, 2 * k + 1) for k in range(n_frag // 2)]
+    ref_nn = [mutual_nn(torch, desc[tdm.PREC_FP32][i], desc[tdm.PREC_FP32][j]) for i, j in pairs]
+    # margins of pair (0, 1) in fp32: relative gap (d2 - d1) / d1 per query
+    d1, d2 = _nn_margins(torch, desc[tdm.PREC_FP32][0], desc[tdm.PREC_FP32][1])
+    gap = ((d2 - d1) / d1.clamp_min(1e-30)).cpu().numpy()
+    edges = [0.0, 1e-4, 1e-3, 3e-3, 1e-2, 3e-2, 1e-1, np.inf]
+    bucket = np.digitize(gap, edges[1:-1])
     vs_fp32 = {}
-    for p, name in ((tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16"), (tdm.PREC_FP8, "fp8")):
-        d16 = torch.cat(desc[p])
-        ab16, m16 = mutual_nn(torch, desc[p][0], desc[p][1])
-        both = m32 & m16 & (ab32 == ab16)
+    for p in list(names)[1:] + ["fp32_alt"]:
+        name = names.get(p, p)
+        dp = torch.cat(desc[p])
+        agree, nn_agree = [], []
+        for (i, j), (ab32, m32) in zip(pairs, ref_nn):
+            abp, mp = mutual_nn(torch, desc[p][i], desc[p][j])
+            agree.append(float((m32 & mp & (ab32 == abp)).sum()) / max(int(m32.sum()), 1))
+            nn_agree.append(float((ab32 == abp).float().mean()))
+        same = (ref_nn[0][0] == mutual_nn(torch, desc[p][0], desc[p][1])[0]).cpu().numpy()
+        pert = (torch.linalg.norm(desc[p][0] - desc[tdm.PREC_FP32][0], dim=1) /
+                torch.linalg.norm(desc[tdm.PREC_FP32][0], dim=1)).cpu().numpy()
         vs_fp32[name] = {
-            "descriptor_rel_l2": float(torch.linalg.norm(d16 - d32) / torch.linalg.norm(d32)),
-            "nn_agreement_pair01": float((ab32 == ab16).float().mean()),
-            "mutual_nn_agreement_pair01": float(both.sum()) / max(int(m32.sum()), 1),
+            "descriptor_rel_l2": float(torch.linalg.norm(dp - d32) / torch.linalg.norm(d32)),
+            "median_per_descriptor_rel_err": float(np.median(pert)),
+            "mutual_nn_agreement_mean": float(np.mean(agree)),
+            "mutual_nn_agreement_min": float(np.min(agree)),
+            "mutual_nn_agreement_pair01": agree[0],
+            "nn_agreement_mean": float(np.mean(nn_agree)),
+            "nn_disagreement_by_margin_pair01": [
+                float(1.0 - same[bucket == b].mean()) if np.any(bucket == b) else None
+                for b in range(len(edges) - 1)],
         }
-    # per-layer errors vs FP64 on the first fragment's first patches
+    margins = {
+        "relative_gap_bucket_edges": [e if np.isfinite(e) else "inf" for e in edges],
+        "queries_per_bucket": [int(np.sum(bucket == b)) for b in range(len(edges) - 1)],
+        "median_relative_gap": float(np.median(gap)),
+    }
     shapes = netdef.layer_shapes()
     p_host = first_patches.cpu().numpy()
     ref = oracle.net_forward_fp64(p_host, dict(weights), shapes)
     per_layer = {}
-    for p, name in ((tdm.PREC_FP32, "fp32"), (tdm.PREC_BF16, "bf16"), (tdm.PREC_FP16, "fp16"),
-                    (tdm.PREC_FP8, "fp8")):
+    for p, name in names.items():
         errs = []
         for i, (_n, _ci, co, _k, _d, od, pooled) in enumerate(shapes):
             e = pooled if pooled is not None else od
@@ -435,9 +504,12 @@ def precision_study(torch, js, wt, weights, n_frag=4, n_kp=5000, layer_patches=8
     for n in nets.values():
         n.close()
     return {
-        "fragments": n_frag, "keypoints_per_fragment": n_kp,
+        "fragments": n_frag, "keypoints_per_fragment": n_kp, "patches": n_frag * n_kp,
+        "pairs": pairs,
         "vs_fp32": vs_fp32,
-        "mutual_nn_pairs_fp32_pair01": int(m32.sum()),
+        "fp32_floor": "fp32_alt: the fp32 net at max_batch 64 (other split-K summation order)",
+        "mutual_nn_pairs_fp32_pair01": int(ref_nn[0][1].sum()),
+        "nn_margins_fp32_pair01": margins,
         "per_layer_rel_l2_vs_fp64": per_layer,
         "per_layer_patches": layer_patches,
     }
@@ -794,8 +866,9 @@ def main() -> int:
                     help="precision of the headline pipeline (default fp32, the plain path)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving sweep (e.g. under a profiler)")
-    ap.add_argument("--only", nargs="+", choices=["tsdf", "tdf"],
-                    help="run only these workloads and print their figures (for counter runs)")
+    ap.add_argument("--only", nargs="+", choices=["tsdf", "tdf", "net"],
+                    help="run only these workloads and print their figures (for counter runs); "
+                         "net: three descriptor passes of --batch keypoints at --precision")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (reference for HBM counters)")
     args = ap.parse_args()
@@ -833,6 +906,9 @@ def main() -> int:
         if "tsdf" in args.only:
             log("tsdf fusion")
             out["tsdf"] = tsdf_bench(torch)
+        if "net" in args.only:
+            log(f"descriptor passes, {args.precision}")
+            out["net"] = net_passes(torch, args.precision, args.batch)
         print(json.dumps(out), flush=True)
         return 0
     log("building inputs")

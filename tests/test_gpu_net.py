@@ -191,6 +191,9 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
         assert torch.equal(got, want), prec
 
 
+NUM_VARIANTS = 9  # conv tile variants (tdm_net_set_layer_variant)
+
+
 def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches):
     """Every tile variant of every convolution gives the same descriptors bit
     for bit, so the autotuner changes speed only."""
@@ -201,7 +204,7 @@ def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches)
         net = _make_net(torch, js, wt, prec, patches[0], max_batch=4)
         ref = torch.empty(len(p), 512, dtype=torch.float32, device="cuda:0")
         net.forward(p, ref)
-        for v in range(9):
+        for v in range(NUM_VARIANTS):
             for i in range(1, len(net.layer_names)):
                 net.set_layer_variant(i, v)  # variants that do not fit fall back
             got = torch.empty_like(ref)
@@ -209,8 +212,31 @@ def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches)
             assert torch.equal(got, ref), (prec, v)
         chosen = net.autotune(4)
         print("autotuned variants", prec, chosen, flush=True)
-        assert all(-1 <= c < 9 for c in chosen)
+        assert all(-1 <= c < NUM_VARIANTS for c in chosen)
         got = torch.empty_like(ref)
         net.forward(p, got)
         assert torch.equal(got, ref)
         net.close()
+
+
+def test_tile_variants_bit_identical_without_split(torch_gpu, net_files):
+    """64 patches at max_batch 64: the early layers run without split-K (conv2
+    with its fused pool, conv3, conv4), so every variant's direct epilogue is
+    compared too."""
+    torch = torch_gpu
+    js, wt, _ = net_files
+    rng = np.random.default_rng(17)
+    p = torch.from_numpy(rng.random((64, 30, 30, 30), dtype=np.float32)).to("cuda:0")
+    for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16):
+        net = tdm.Net(js, prec, max_batch=64)
+        net.load_weights(wt)
+        ref = torch.empty(64, 512, dtype=torch.float32, device="cuda:0")
+        net.forward(p, ref)
+        for v in range(NUM_VARIANTS):
+            for i in range(1, len(net.layer_names)):
+                net.set_layer_variant(i, v)
+            got = torch.empty_like(ref)
+            net.forward(p, got)
+            assert torch.equal(got, ref), (prec, v)
+        net.close()
+

@@ -212,3 +212,31 @@ def test_accurate_tdf_bit_exact(torch_gpu):
     inside = np.all((gi >= 0) & (gi < np.array(dims)), axis=1)
     assert 0 < inside.sum() < len(cloud)
     assert not np.array_equal(oracle.tdf_accurate(cloud[inside], dims, origin, vs, tr), want)
+
+
+@pytest.mark.parametrize("trunc_vox", [5, 12, 30])
+def test_accurate_tdf_dense_and_wide_reach(torch_gpu, trunc_vox):
+    """Dense clusters put thousands of candidates on a tile (several 512-point
+    chunks) and wide truncations walk 5^3 and 9^3 bins: still the oracle's
+    values bit for bit, and voxels out of reach read 0."""
+    torch = torch_gpu
+    rng = np.random.default_rng(100 + trunc_vox)
+    vs = np.float32(0.01)
+    tr = np.float32(vs * np.float32(trunc_vox))
+    wide = trunc_vox > 20  # keep the brute-force oracle to seconds
+    dims = [26, 22, 20] if wide else [44, 38, 36]
+    origin = np.array([0.1, -0.2, 0.3], np.float32)
+    dense = origin + (np.array([10, 12, 9]) + rng.random((1500 if wide else 6000, 3)) * 6
+                      ).astype(np.float32) * vs
+    sparse = origin + (rng.random((400 if wide else 1500, 3)) * np.array([60, 50, 50]) - 8
+                       ).astype(np.float32) * vs
+    cloud = np.concatenate([dense, sparse]).astype(np.float32)
+    print(f"case accurate dense trunc={trunc_vox} voxels", flush=True)
+    want = oracle.tdf_accurate(cloud, dims, origin, vs, tr)
+    got = torch.full((dims[2], dims[1], dims[0]), -1.0, device="cuda:0")
+    tdm.tdf_accurate(torch.from_numpy(cloud).to("cuda:0"), dims, origin, vs, tr, got)
+    torch.cuda.synchronize()
+    g = got.cpu().numpy()
+    assert np.array_equal(g.view(np.uint32), want.view(np.uint32)), np.abs(g - want).max()
+    if trunc_vox == 5:
+        assert np.count_nonzero(want == 0) > 0 and np.count_nonzero(want) > 1000
