@@ -138,11 +138,15 @@ struct Mfma<f16_t> {
     }
 };
 
+// Round to nearest even: gfx950 converts in one instruction
+// (v_cvt_pk_bf16_f32), bit-identical to the integer rounding of
+// float_to_bf16 on every non-NaN input (checked over all 2^32 patterns on
+// the device); a NaN stays a NaN.
 __device__ inline bf16_t to_bf16(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (bf16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (bf16_t)(u >> 16);
+    const __bf16 h = (__bf16)f;
+    bf16_t u;
+    __builtin_memcpy(&u, &h, 2);
+    return u;
 }
 
 __device__ inline float from_bf16(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
@@ -757,10 +761,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
             vb[i] = a.base + (b * (long)a.P * a.P * a.P + (long)pz * sz[i] + (long)py * sy[i] + px);
         }
     }
-    // Input value of tap t (< 27) for row block i.  Outside the grid reads 0;
-    // the address is formed only for taps inside it.
-    auto tap_value = [&](int i, int t) -> float {
-        const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+    // Input value of the tap (kz, ky, kx) for row block i.  Outside the grid
+    // reads 0; the address is formed only for taps inside it.
+    auto tap_value = [&](int i, int kz, int ky, int kx) -> float {
         if (!GRID || inb[i]) return vb[i][kz * sz[i] + ky * sy[i] + kx];
         const int x = gx[i] + kx, y = gy[i] + ky, z = gz[i] + kz;
         if (x < 0 || y < 0 || z < 0 || x >= ex[i] || y >= ey[i] || z >= ez[i]) return 0.0f;
@@ -774,13 +777,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
 
     const T *__restrict__ w = static_cast<const T *>(a.w);
     if constexpr (F32) {
-        // 7 k-steps of 4 taps: lane group g supplies tap 4s + g.
+        // 7 k-steps of 4 taps: lane group g supplies tap 4s + g.  The tap
+        // number depends on the lane, so its (kz, ky, kx) is decoded once
+        // here, not once per row block.
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             const int t = 4 * s + g;
+            const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
             float av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = t < 27 ? tap_value(i, t) : 0.0f;
+            for (int i = 0; i < 4; ++i) av[i] = t < 27 ? tap_value(i, kz, ky, kx) : 0.0f;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float *wf = reinterpret_cast<const float *>(w);
@@ -793,7 +799,17 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
     } else {
-        // One k-step of 32 taps: lane group g supplies taps 8g .. 8g+7.
+        // One k-step of 32 taps: lane group g supplies taps 8g .. 8g+7.  The
+        // tap numbers depend on the lane, so their (kz, ky, kx) are decoded
+        // once here, not once per row block.
+        int tk[8][3];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = 8 * g + e;
+            tk[e][0] = t / 9;
+            tk[e][1] = (t / 3) % 3;
+            tk[e][2] = t % 3;
+        }
         uint4 af[4], bf[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -801,7 +817,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int t = 8 * g + e;
-                hv[e] = bits16<T>(t < 27 ? tap_value(i, t) : 0.0f);
+                hv[e] = bits16<T>(t < 27 ? tap_value(i, tk[e][0], tk[e][1], tk[e][2]) : 0.0f);
             }
             __builtin_memcpy(&af[i], hv, 16);
         }
