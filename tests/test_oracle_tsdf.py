@@ -156,3 +156,32 @@ def test_surface_points_oracle_matches_literal_loop():
     want = np.array(want, np.float32)
     assert len(want) > 10
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_surface_points_closed_form():
+    """Planted voxels on a grid whose coordinates are exact in float
+    (origin and voxel size dyadic): the extracted points are exactly the
+    planted voxels that pass |tsdf| < 0.2 and weight > 1, in linear index
+    order, at origin + index * voxel_size.  The threshold edges are planted
+    too: 0.2, -0.2 and weight 1 are out, the floats just inside are in."""
+    dims = (9, 8, 7)
+    tsdf = np.ones((dims[2], dims[1], dims[0]), np.float32)
+    weight = np.zeros_like(tsdf)
+    inside_t = np.nextafter(np.float32(0.2), np.float32(0))
+    inside_w = np.nextafter(np.float32(1.0), np.float32(2))
+    planted = [  # (x, y, z, tsdf, weight, kept)
+        (8, 7, 6, 0.0, 5.0, True), (0, 0, 0, -0.1, 2.0, True), (3, 2, 1, inside_t, 3.0, True),
+        (4, 4, 4, -inside_t, inside_w, True), (5, 1, 2, 0.2, 4.0, False),
+        (6, 1, 2, -0.2, 4.0, False), (2, 5, 3, 0.05, 1.0, False), (1, 6, 5, 0.3, 9.0, False),
+        (7, 3, 0, -0.19, 1.5, True)]
+    for x, y, z, t, w, _ in planted:
+        tsdf[z, y, x] = np.float32(t)
+        weight[z, y, x] = np.float32(w)
+    origin = np.array([-2.0, 0.5, 1.25], np.float32)
+    vs = np.float32(0.25)
+    got = oracle.surface_points(tsdf, weight, origin, vs)
+    kept = sorted([p for p in planted if p[5]], key=lambda p: (p[2] * dims[1] + p[1]) * dims[0] + p[0])
+    want = np.array([[origin[0] + 0.25 * x, origin[1] + 0.25 * y, origin[2] + 0.25 * z]
+                     for x, y, z, *_ in kept], np.float64)
+    assert got.shape == (5, 3)
+    assert np.array_equal(got.astype(np.float64), want)
