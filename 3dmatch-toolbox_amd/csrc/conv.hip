@@ -623,6 +623,162 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
     conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
 }
 
+// s_waitcnt with vmcnt(n) and no wait on the other counters (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ inline void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Direct-to-LDS staging.  The same tiles, swizzled layout and k order as
+// conv_lds_kernel, but each stage is copied from global memory into LDS by
+// global_load_lds_dwordx4: one wave instruction writes 64 consecutive 16-byte
+// slots, each lane fetching the global piece that belongs at its slot, so no
+// data passes through VGPRs and no ds_write is issued (the staged form spends
+// about a third of its LDS instructions on those writes).  A ring of NST
+// stage buffers keeps NST-1 stages of loads in flight: at each stage a wave
+// waits for its own loads of that stage (vmcnt), the barrier makes everyone's
+// visible and frees the buffer read at the previous stage, which then
+// receives the loads of stage s+NST-1.  The buffers are separate LDS arrays
+// indexed at compile time (the loop is unrolled NST times), so the compiler
+// can see that the reads of one stage do not depend on the copies in flight
+// into another.
+template <typename T, typename OT, int WMW, int WTM, int WTN, int STAGE, int NST, bool POOL,
+          bool SPLIT>
+__global__ __launch_bounds__(256) void conv_dma_kernel(ConvArgs p) {
+    using TL = Tile<WMW, WTM, WTN, STAGE>;
+    constexpr int BM = TL::BM, BN = TL::BN, WNW = TL::WNW;
+    constexpr int PCS = STAGE / 16, HALVES = STAGE / 64;
+    constexpr int IB = WTM / 16, JB = WTN / 16;
+    constexpr int SK = STAGE / (int)sizeof(T);
+    constexpr int SLOTS = (BM + BN) * PCS;  // 16-byte slots per stage
+    constexpr int J = SLOTS / 256;          // load instructions per wave per stage
+    static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0, "stage layout");
+    static_assert(NST == 2 || NST == 3, "ring depth");
+    static_assert((NST - 2) * J < 64, "vmcnt range");
+    constexpr int SB = (BM + BN) * STAGE;
+    __shared__ __attribute__((aligned(16))) unsigned char L0[SB];
+    __shared__ __attribute__((aligned(16))) unsigned char L1[SB];
+    __shared__ __attribute__((aligned(16))) unsigned char L2[NST > 2 ? SB : 16];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WNW, wn = wave % WNW;
+    unsigned tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const long mb = (long)tx * BM;
+    const int nb = ty * BN;
+    const int C = p.C, D = p.D, ks = p.ksize;
+    const long K = (long)p.taps * C;
+    const unsigned char *__restrict__ in = static_cast<const unsigned char *>(p.in);
+    const unsigned char *__restrict__ w = static_cast<const unsigned char *>(p.w);
+
+    // load j of this wave fills slots (4j + wave)*64 + lane of a stage buffer
+    const unsigned char *src[J];
+    bool isa[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int slot = (4 * j + wave) * 64 + lane;
+        const int row = slot / PCS, pl = slot % PCS;
+        const int ps = pl ^ swz<STAGE>(row);
+        isa[j] = row < BM;
+        if (row < BM) {
+            const long m = min(mb + row, p.M - 1);
+            src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + ps * 16;
+        } else {
+            src[j] = w + ((long)(nb + row - BM) * K) * (long)sizeof(T) + ps * 16;
+        }
+    }
+    const int r16 = lane & 15, g = lane >> 4;
+    int a_off[IB][HALVES], b_off[JB][HALVES];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+        const int row = wm * WTM + 16 * i + r16;
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h)
+            a_off[i][h] = row * STAGE + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int row = BM + wn * WTN + 16 * j + r16;
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h)
+            b_off[j][h] = row * STAGE + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
+    }
+
+    const int cps = C / SK;
+    const int ns = p.taps * cps;
+    int s0 = 0, s1 = ns;
+    if (SPLIT) {
+        constexpr int CG = 128 / STAGE;
+        const int nc = ns / CG;
+        s0 = (int)((long)nc * tz / p.splits) * CG;
+        s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
+    }
+    int ftap = s0 / cps, fc0 = (s0 - ftap * cps) * SK;  // the next stage to copy
+    auto issue = [&](unsigned char *buf) {
+        int kz = 0, ky = 0, kx = 0;
+        if (ks == 3) {
+            kz = ftap / 9;
+            ky = (ftap / 3) % 3;
+            kx = ftap % 3;
+        }
+        const long aoff = ((long)((kz * D + ky) * D + kx) * C + fc0) * (long)sizeof(T);
+        const long boff = ((long)ftap * C + fc0) * (long)sizeof(T);
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src[j] + (isa[j] ? aoff : boff)),
+                (__attribute__((address_space(3))) void *)(buf + (4 * j + wave) * 1024), 16, 0, 0);
+        fc0 += SK;
+        if (fc0 == C) {
+            fc0 = 0;
+            ++ftap;
+        }
+    };
+
+    floatx4 acc[IB][JB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto compute = [&](const unsigned char *buf) {
+#pragma unroll
+        for (int h = 0; h < HALVES; ++h) {
+            uint4 a[IB], b[JB];
+#pragma unroll
+            for (int i = 0; i < IB; ++i) a[i] = *reinterpret_cast<const uint4 *>(buf + a_off[i][h]);
+#pragma unroll
+            for (int j = 0; j < JB; ++j) b[j] = *reinterpret_cast<const uint4 *>(buf + b_off[j][h]);
+            Mfma<T>::chunk(acc, a, b);
+        }
+    };
+    // stage s in buffer `cur`; the copies of stage s+NST-1 go to `nxt` (the
+    // buffer of stage s-1)
+    auto step = [&](int s, unsigned char *cur, unsigned char *nxt) {
+        if (NST == 3 && s + 1 < s1) wait_vmcnt<(NST - 2) * J>();
+        else wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        if (s + NST - 1 < s1) issue(nxt);
+        compute(cur);
+    };
+    if (s0 < s1) {
+        issue(L0);
+        if (NST == 3 && s0 + 1 < s1) issue(L1);
+        for (int s = s0; s < s1; s += NST) {
+            if (NST == 3) {
+                step(s, L0, L2);
+                if (s + 1 < s1) step(s + 1, L1, L0);
+                if (s + 2 < s1) step(s + 2, L2, L1);
+            } else {
+                step(s, L0, L1);
+                if (s + 1 < s1) step(s + 1, L1, L0);
+            }
+        }
+    }
+    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+}
+
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
 template <typename OT>
 __global__ void split_reduce_kernel(const float *__restrict__ part, int splits, long MN, int N,
@@ -971,7 +1127,23 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         else                                                                                  \
             conv_lds_kernel<T, OT, WMW, WTM, WTN, ST, false, false><<<grid, 256, 0, s>>>(p);  \
     } while (0)
+#define TDM_CONV_DMA(WMW, WTM, WTN, ST, NS)                                                   \
+    do {                                                                                      \
+        using TL = Tile<WMW, WTM, WTN, ST>;                                                   \
+        dim3 grid((unsigned)((p.M + TL::BM - 1) / TL::BM), d.N / TL::BN, splits);             \
+        if (splits > 1)                                                                       \
+            conv_dma_kernel<T, OT, WMW, WTM, WTN, ST, NS, false, true><<<grid, 256, 0, s>>>(p); \
+        else if (d.pool)                                                                      \
+            conv_dma_kernel<T, OT, WMW, WTM, WTN, ST, NS, true, false><<<grid, 256, 0, s>>>(p); \
+        else                                                                                  \
+            conv_dma_kernel<T, OT, WMW, WTM, WTN, ST, NS, false, false><<<grid, 256, 0, s>>>(p); \
+    } while (0)
         switch (v) {
+        case 9: TDM_CONV_DMA(4, 64, 64, 64, 3); break;
+        case 10: TDM_CONV_DMA(2, 64, 64, 64, 3); break;
+        case 11: TDM_CONV_DMA(2, 128, 32, 64, 3); break;
+        case 12: TDM_CONV_DMA(2, 64, 64, 128, 2); break;
+        case 13: TDM_CONV_DMA(4, 64, 64, 128, 2); break;
         case 0: TDM_CONV_LDS(2, 64, 32, 128); break;
         case 1: TDM_CONV_LDS(2, 64, 64, 128); break;
         case 2: TDM_CONV_LDS(4, 64, 64, 128); break;
@@ -983,6 +1155,7 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         default: TDM_CONV_LDS(2, 128, 32, 64); break;
         }
 #undef TDM_CONV_LDS
+#undef TDM_CONV_DMA
     } else if constexpr (sizeof(T) == 1) {
         return tdm::fail(TDM_ERR_UNSUPPORTED, "fp8 convolution with %d input channels", d.C);
     } else {
@@ -1042,12 +1215,14 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
     return taps * d.C / (dt == DType::F32 ? 16 : 32);
 }
 
-// Tile variants of the LDS kernel: {waves along m, wave tile m, wave tile n,
-// stage bytes}.
+// Tile variants: {waves along m, wave tile m, wave tile n, stage bytes};
+// 0-8 stage through VGPRs (conv_lds_kernel), 9-13 copy straight into LDS
+// (conv_dma_kernel; 9-11 with three stage buffers, 12-13 with two).
 namespace {
 constexpr int kVariants[kConvVariants][4] = {
     {2, 64, 32, 128},  {2, 64, 64, 128}, {4, 64, 64, 128}, {2, 128, 32, 128}, {2, 128, 64, 128},
-    {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64}};
+    {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64},  {4, 64, 64, 64},
+    {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}};
 }
 
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {

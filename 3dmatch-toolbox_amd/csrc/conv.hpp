@@ -46,7 +46,7 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
 
 // Tile variants of the LDS kernel (0 .. kConvVariants-1): whether one fits
 // a layer (its N is a multiple of the tile's BN), the default, and the tile.
-constexpr int kConvVariants = 9;
+constexpr int kConvVariants = 14;
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d);
 int conv_default_variant(DType dt, const ConvDesc &d);
 void conv_variant_tile(int v, int *bm, int *bn);
