@@ -868,9 +868,39 @@ __global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
         if (slot[k] >= 0) binned[T.cnt[slot[k]] + rank[k]] = p[k];
 }
 
-// One 8x8x8 tile per workgroup of 512 threads.  The grid is zeroed first (a
-// full-line fill), and a voxel's value is 0 whenever no point lies within
-// the truncation distance, so only voxels with a point in reach are stored.
+// The tiles with a point within reach: tile t is listed when one of the
+// bins within `reach` tiles of it holds a point (offsets from the bin scan).
+// One atomic per wave (ballot) appends the wave's tiles to `list`; the
+// order of the list does not matter, tiles are independent.
+__global__ __launch_bounds__(256) void acc_near_kernel(const int *__restrict__ offsets, int tx,
+                                                       int ty, int tz, int reach,
+                                                       int *__restrict__ count,
+                                                       int *__restrict__ list) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nt = (long)tx * ty * tz;
+    bool near = false;
+    if (t < nt) {
+        const int bx = (int)(t % tx), by = (int)((t / tx) % ty), bz = (int)(t / ((long)tx * ty));
+        for (int z = max(0, bz - reach); z <= min(tz - 1, bz + reach) && !near; ++z)
+            for (int y = max(0, by - reach); y <= min(ty - 1, by + reach) && !near; ++y) {
+                const long row = ((long)z * ty + y) * tx;
+                const int x0 = max(0, bx - reach), x1 = min(tx - 1, bx + reach);
+                near = offsets[row + x1 + 1] > offsets[row + x0];  // any point in the run
+            }
+    }
+    const unsigned long long m = __ballot(near);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    if (near) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)t;
+}
+
+// 8x8x8 tiles, one per workgroup of 512 threads, over the list of tiles with
+// a point in reach.  The grid is zeroed first (a full-line fill),
+// and a voxel's value is 0 whenever no point lies within the truncation
+// distance, so only those tiles are visited and only voxels with a point in
+// reach are stored.
 // The candidates are the points of the tiles within `reach` tiles (reach*8
 // voxels >= the truncation radius plus the half voxel a point may sit off
 // its voxel): the workgroup reads the (2 reach + 1)^3 bin ranges at once,
@@ -882,17 +912,12 @@ __global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
 // at least the box's, and the threshold trunc^2 (1 + 2^-20) clears the
 // rounding of trunc^2 itself.  A tile with no candidate returns at once.
 constexpr int kAccMaxBins = 9 * 9 * 9;  // (2 reach + 1)^3 with reach <= 4
-__global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
-                                                      const int *__restrict__ offsets, int dx,
-                                                      int dy, int dz, int tx, int ty, int tz,
-                                                      int reach, float ox, float oy, float oz,
-                                                      float vs, float trunc,
-                                                      float *__restrict__ tdf) {
-    __shared__ float4 sp[512];
-    __shared__ int bstart[kAccMaxBins];
-    __shared__ int bpre[kAccMaxBins + 1];
-    __shared__ int wave_n[8];
-    const int t = blockIdx.x;
+
+// One tile of acc_tdf_kernel (all 512 threads of the workgroup call it).
+__device__ inline void acc_tile(const float4 *__restrict__ binned, const int *__restrict__ offsets,
+                                int t, int dx, int dy, int dz, int tx, int ty, int tz, int reach,
+                                float ox, float oy, float oz, float vs, float trunc, float4 *sp,
+                                int *bstart, int *bpre, int *wave_n, float *__restrict__ tdf) {
     const int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
     const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
     const int x = bx * kAT + lx, y = by * kAT + ly, z = bz * kAT + lz;
@@ -926,7 +951,7 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
     }
     __syncthreads();
     const int total = bpre[nbins];
-    if (total == 0) return;  // uniform: nothing within reach, the grid holds 0
+    if (total == 0) return;  // uniform: nothing within reach, the grid holds 0 here
     // voxel centre (origin is the centre of voxel 0)
     const float cx = ox + (float)x * vs, cy = oy + (float)y * vs, cz = oz + (float)z * vs;
     const float lox = ox + (float)(bx * kAT) * vs, hix = ox + (float)(bx * kAT + kAT - 1) * vs;
@@ -975,6 +1000,27 @@ __global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__
     const float d = sqrtf(best);  // sqrt is monotone: the minimum of the distances
     const float v = 1.0f - fminf(d, trunc) / trunc;
     if (v != 0.0f) tdf[((long)z * dy + y) * dx + x] = v;  // 0 is already there
+}
+
+__global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
+                                                      const int *__restrict__ offsets, int dx,
+                                                      int dy, int dz, int tx, int ty, int tz,
+                                                      int reach, float ox, float oy, float oz,
+                                                      float vs, float trunc,
+                                                      const int *__restrict__ near_count,
+                                                      const int *__restrict__ near_list,
+                                                      float *__restrict__ tdf) {
+    __shared__ float4 sp[512];
+    __shared__ int bstart[kAccMaxBins];
+    __shared__ int bpre[kAccMaxBins + 1];
+    __shared__ int wave_n[8];
+    // workgroup b takes entry b of the list of tiles with a point in reach;
+    // the workgroups past its end exit at once (their tiles keep the zeros
+    // of the fill).  One workgroup per tile leaves the balancing of uneven
+    // tiles to the hardware dispatcher.
+    if ((int)blockIdx.x >= *near_count) return;
+    acc_tile(binned, offsets, near_list[blockIdx.x], dx, dy, dz, tx, ty, tz, reach, ox, oy, oz, vs,
+             trunc, sp, bstart, bpre, wave_n, tdf);
 }
 
 int check_dims(int dx, int dy, int dz) {
@@ -1340,18 +1386,21 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     // points binned up to this many voxels outside the grid (>= reach_vox)
     const int margin = reach * kAT;
     const long nb = (nt + 1023) / 1024;  // scan blocks
-    const size_t bytes = sizeof(int) * (2 * (size_t)nt + 1 + nb) + 16 + sizeof(float4) * (size_t)num_pts;
+    const size_t bytes = sizeof(int) * (3 * (size_t)nt + 2 + nb) + 32 + sizeof(float4) * (size_t)num_pts;
     void *area = nullptr;
     TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &area, s));
     int *counts = static_cast<int *>(area);
-    int *offsets = counts + nt;  // nt + 1 entries
+    int *near_count = counts + nt;  // 1 entry, zeroed with the counts
+    int *offsets = near_count + 1;  // nt + 1 entries
     int *bsum = offsets + nt + 1;  // nb entries
     float4 *binned = reinterpret_cast<float4 *>(
         (reinterpret_cast<uintptr_t>(bsum + nb) + 15) & ~uintptr_t(15));
+    int *near_list = reinterpret_cast<int *>(binned + num_pts);  // nt entries
     const unsigned fb = (unsigned)std::min<long>((nt + 255) / 256, 16384);
-    // one launch zeroes the output grid (full lines) and the bin counts
+    // one launch zeroes the output grid (full lines), the bin counts and the
+    // near-tile count
     clear_regions(tdf, (size_t)dims[0] * dims[1] * dims[2], reinterpret_cast<unsigned char *>(counts),
-                  sizeof(int) * (size_t)nt, s);
+                  sizeof(int) * ((size_t)nt + 1), s);
     TDM_LAUNCHED();
     if (num_pts > 0) {
         acc_count_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
@@ -1374,9 +1423,12 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
         TDM_LAUNCHED();
     }
     TDM_CHECK(nt < (1L << 31), TDM_ERR_INVALID_ARG, "grid too large");
-    acc_tdf_kernel<<<(unsigned)nt, 512, 0, s>>>(binned, offsets, dims[0], dims[1], dims[2], tx, ty,
-                                                tz, reach, origin[0], origin[1], origin[2],
-                                                voxel_size, trunc_margin, tdf);
+    acc_near_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(offsets, tx, ty, tz, reach,
+                                                                 near_count, near_list);
+    TDM_LAUNCHED();
+    acc_tdf_kernel<<<(unsigned)nt, 512, 0, s>>>(
+        binned, offsets, dims[0], dims[1], dims[2], tx, ty, tz, reach, origin[0], origin[1],
+        origin[2], voxel_size, trunc_margin, near_count, near_list, tdf);
     TDM_LAUNCHED();
     return TDM_OK;
 }
