@@ -868,66 +868,107 @@ __global__ __launch_bounds__(kBinThreads) void acc_fill_kernel(
         if (slot[k] >= 0) binned[T.cnt[slot[k]] + rank[k]] = p[k];
 }
 
-// The tiles with a point within reach: tile t is listed when one of the
-// bins within `reach` tiles of it holds a point (offsets from the bin scan).
-// One atomic per wave (ballot) appends the wave's tiles to `list`; the
-// order of the list does not matter, tiles are independent.
+// Candidates per work item of acc_tdf_kernel: a tile whose neighbour bins
+// hold more points is split into segments of this many candidates, each a
+// work item of its own (the points of a fragment crowd near its camera: the
+// heaviest tile of a 200k-point view has ~50x the median's candidates).
+constexpr int kAccSeg = 1024;
+
+// The work items of acc_tdf_kernel: (tile, segment) for every segment of
+// kAccSeg candidates of the tiles with a point within reach (candidates =
+// the points of the bins within `reach` tiles, from the offsets of the bin
+// scan).  A wave scan numbers the items of the wave's tiles, and one atomic
+// per wave reserves their places in `list`; the order of the list does not
+// matter, the items are independent.
 __global__ __launch_bounds__(256) void acc_near_kernel(const int *__restrict__ offsets, int tx,
                                                        int ty, int tz, int reach,
                                                        int *__restrict__ count,
-                                                       int *__restrict__ list) {
+                                                       int2 *__restrict__ list) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     const long nt = (long)tx * ty * tz;
-    bool near = false;
+    int total = 0;
     if (t < nt) {
         const int bx = (int)(t % tx), by = (int)((t / tx) % ty), bz = (int)(t / ((long)tx * ty));
-        for (int z = max(0, bz - reach); z <= min(tz - 1, bz + reach) && !near; ++z)
-            for (int y = max(0, by - reach); y <= min(ty - 1, by + reach) && !near; ++y) {
+        const int x0 = max(0, bx - reach), x1 = min(tx - 1, bx + reach);
+        for (int z = max(0, bz - reach); z <= min(tz - 1, bz + reach); ++z)
+            for (int y = max(0, by - reach); y <= min(ty - 1, by + reach); ++y) {
                 const long row = ((long)z * ty + y) * tx;
-                const int x0 = max(0, bx - reach), x1 = min(tx - 1, bx + reach);
-                near = offsets[row + x1 + 1] > offsets[row + x0];  // any point in the run
+                total += offsets[row + x1 + 1] - offsets[row + x0];  // the points of the run
             }
     }
-    const unsigned long long m = __ballot(near);
+    const int n = (total + kAccSeg - 1) / kAccSeg;
     const int lane = threadIdx.x & 63;
+    int v = n;  // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
     int base = 0;
-    if (lane == 0 && m) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0);
-    if (near) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int)t;
+    if (lane == 63 && v > 0) base = atomicAdd(count, v);
+    base = __shfl(base, 63) + v - n;
+    for (int k = 0; k < n; ++k) list[base + k] = make_int2((int)t, k);
 }
 
-// 8x8x8 tiles, one per workgroup of 512 threads, over the list of tiles with
-// a point in reach.  The grid is zeroed first (a full-line fill),
-// and a voxel's value is 0 whenever no point lies within the truncation
-// distance, so only those tiles are visited and only voxels with a point in
-// reach are stored.
+// 8x8x8 tiles, one per workgroup of 256 threads, over the list of tiles with
+// a point in reach.  The grid is zeroed first (a full-line fill), and a
+// voxel's value is 0 whenever no point lies within the truncation distance,
+// so only those tiles are visited and only voxels with a point in reach are
+// stored.
 // The candidates are the points of the tiles within `reach` tiles (reach*8
 // voxels >= the truncation radius plus the half voxel a point may sit off
 // its voxel): the workgroup reads the (2 reach + 1)^3 bin ranges at once,
 // numbers their points consecutively (prefix sums in LDS), and streams them
-// 512 at a time through LDS, keeping only those that can be within the
-// truncation distance of one of the tile's voxel centres.  The test uses the
-// box of the centres, computed with the same float operations as the centres
-// (so it contains them); float rounding is monotone, so every centre's d^2 is
-// at least the box's, and the threshold trunc^2 (1 + 2^-20) clears the
-// rounding of trunc^2 itself.  A tile with no candidate returns at once.
+// 256 at a time through LDS, keeping only those that can be within the
+// truncation distance of one of the tile's voxel centres.  Each wave then
+// keeps, in a list of its own, the ones within reach of its 4x4x8 block of
+// the tile, and each lane takes two voxels of that block, (x, y, z) and
+// (x, y, z + 4), which share the x and y terms of the distance.  The tests
+// use the box of the centres, computed with the same float operations as
+// the centres (so it contains them); float rounding is monotone, so every
+// centre's d^2 is at least the box's, and the threshold trunc^2 (1 + 2^-20)
+// clears the rounding of trunc^2 itself.  A tile with no candidate returns
+// at once.
 constexpr int kAccMaxBins = 9 * 9 * 9;  // (2 reach + 1)^3 with reach <= 4
+constexpr int kAccThreads = 256;
 
-// One tile of acc_tdf_kernel (all 512 threads of the workgroup call it).
+struct AccBox {
+    float lox, hix, loy, hiy, loz, hiz;
+    // squared distance from q to the box, summed as the voxel distances are
+    __device__ float d2(const float4 &q) const {
+        const float ex = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.0f);
+        const float ey = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.0f);
+        const float ez = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.0f);
+        return ex * ex + ey * ey + ez * ez;
+    }
+};
+
+// The box of the centres of voxels [x0, x0 + nx) x [y0, y0 + ny) x [z0, z0 + nz).
+__device__ inline AccBox acc_box(int x0, int y0, int z0, int nx, int ny, int nz, float ox,
+                                 float oy, float oz, float vs) {
+    return {ox + (float)x0 * vs,        ox + (float)(x0 + nx - 1) * vs,
+            oy + (float)y0 * vs,        oy + (float)(y0 + ny - 1) * vs,
+            oz + (float)z0 * vs,        oz + (float)(z0 + nz - 1) * vs};
+}
+
+// One work item of acc_tdf_kernel: segment `seg` of tile t's candidates (all
+// 256 threads of the workgroup call it).
 __device__ inline void acc_tile(const float4 *__restrict__ binned, const int *__restrict__ offsets,
-                                int t, int dx, int dy, int dz, int tx, int ty, int tz, int reach,
+                                int t, int seg, int dx, int dy, int dz, int tx, int ty, int tz,
+                                int reach,
                                 float ox, float oy, float oz, float vs, float trunc, float4 *sp,
-                                int *bstart, int *bpre, int *wave_n, float *__restrict__ tdf) {
+                                float4 *wsp, int *bstart, int *bpre, int *wave_n,
+                                float *__restrict__ tdf) {
     const int bx = t % tx, by = (t / tx) % ty, bz = t / (tx * ty);
-    const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
-    const int x = bx * kAT + lx, y = by * kAT + ly, z = bz * kAT + lz;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wave w: x half (w & 1), y half (w >> 1), all of z; lane: 4x4x4
+    const int sx = bx * kAT + (wave & 1) * 4, sy = by * kAT + (wave >> 1) * 4, sz = bz * kAT;
+    const int x = sx + (lane & 3), y = sy + ((lane >> 2) & 3), z = sz + (lane >> 4);
     const int x0 = max(0, bx - reach), y0 = max(0, by - reach), z0 = max(0, bz - reach);
     const int nx = min(tx - 1, bx + reach) - x0 + 1, ny = min(ty - 1, by + reach) - y0 + 1,
               nz = min(tz - 1, bz + reach) - z0 + 1;
     const int nbins = nx * ny * nz;
     // bin ranges, then their exclusive prefix (wave 0, 64 bins per step)
-    for (int k = threadIdx.x; k < nbins; k += 512) {
+    for (int k = threadIdx.x; k < nbins; k += kAccThreads) {
         const int kx = k % nx, ky = (k / nx) % ny, kz = k / (nx * ny);
         const long nt = ((long)(z0 + kz) * ty + (y0 + ky)) * tx + (x0 + kx);
         const int a = offsets[nt];
@@ -951,19 +992,22 @@ __device__ inline void acc_tile(const float4 *__restrict__ binned, const int *__
     }
     __syncthreads();
     const int total = bpre[nbins];
-    if (total == 0) return;  // uniform: nothing within reach, the grid holds 0 here
-    // voxel centre (origin is the centre of voxel 0)
-    const float cx = ox + (float)x * vs, cy = oy + (float)y * vs, cz = oz + (float)z * vs;
-    const float lox = ox + (float)(bx * kAT) * vs, hix = ox + (float)(bx * kAT + kAT - 1) * vs;
-    const float loy = oy + (float)(by * kAT) * vs, hiy = oy + (float)(by * kAT + kAT - 1) * vs;
-    const float loz = oz + (float)(bz * kAT) * vs, hiz = oz + (float)(bz * kAT + kAT - 1) * vs;
+    const int c_lo = seg * kAccSeg, c_hi = min(total, c_lo + kAccSeg);
+    if (c_lo >= c_hi) return;  // uniform: nothing within reach, the grid holds 0 here
+    // voxel centres (origin is the centre of voxel 0)
+    const float cx = ox + (float)x * vs, cy = oy + (float)y * vs;
+    const float cz0 = oz + (float)z * vs, cz1 = oz + (float)(z + 4) * vs;
+    const AccBox tile = acc_box(bx * kAT, by * kAT, bz * kAT, kAT, kAT, kAT, ox, oy, oz, vs);
+    const AccBox mine = acc_box(sx, sy, sz, 4, 4, kAT, ox, oy, oz, vs);
     const float lim = trunc * trunc * (1.0f + 0x1p-20f);
-    float best = __builtin_huge_valf();
-    for (int c0 = 0; c0 < total; c0 += 512) {
+    const unsigned long long below = (1ull << lane) - 1ull;
+    float4 *const wl = wsp + wave * kAccThreads;
+    float best0 = __builtin_huge_valf(), best1 = __builtin_huge_valf();
+    for (int c0 = c_lo; c0 < c_hi; c0 += kAccThreads) {
         const int c = c0 + (int)threadIdx.x;
         bool keep = false;
         float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (c < total) {
+        if (c < c_hi) {
             // the bin holding candidate c: last k with bpre[k] <= c
             int lo = 0, hi = nbins - 1;
             while (lo < hi) {
@@ -972,55 +1016,88 @@ __device__ inline void acc_tile(const float4 *__restrict__ binned, const int *__
                 else hi = mid - 1;
             }
             q = binned[bstart[lo] + (c - bpre[lo])];
-            const float ex = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.0f);
-            const float ey = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.0f);
-            const float ez = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.0f);
-            keep = ex * ex + ey * ey + ez * ez <= lim;
+            keep = tile.d2(q) <= lim;
         }
         const unsigned long long m = __ballot(keep);
         __syncthreads();  // the previous chunk's reads of sp are done
         if (lane == 0) wave_n[wave] = __popcll(m);
         __syncthreads();
-        int pos = __popcll(m & ((1ull << lane) - 1ull)), kept = 0;
-        for (int w = 0; w < 8; ++w) {
+        int pos = __popcll(m & below), kept = 0;
+        for (int w = 0; w < kAccThreads / 64; ++w) {
             const int cnt = wave_n[w];
             pos += w < wave ? cnt : 0;
             kept += cnt;
         }
         if (keep) sp[pos] = q;
         __syncthreads();
-        for (int k = 0; k < kept; ++k) {
-            const float4 p = sp[k];
-            const float ex = cx - p.x, ey = cy - p.y, ez = cz - p.z;
-            const float d2 = ex * ex + ey * ey + ez * ez;
-            best = fminf(best, d2);
+        // this wave's candidates: those within reach of its block
+        int wn = 0;
+        for (int i0 = 0; i0 < kept; i0 += 64) {
+            const int i = i0 + lane;
+            bool k2 = false;
+            float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i < kept) {
+                p = sp[i];
+                k2 = mine.d2(p) <= lim;
+            }
+            const unsigned long long m2 = __ballot(k2);
+            if (k2) wl[wn + __popcll(m2 & below)] = p;
+            wn += __popcll(m2);
+        }
+        __syncthreads();  // the lists are written
+        for (int k = 0; k < wn; ++k) {
+            const float4 p = wl[k];
+            const float ex = cx - p.x, ey = cy - p.y;
+            const float exy = ex * ex + ey * ey;
+            const float ez0 = cz0 - p.z, ez1 = cz1 - p.z;
+            best0 = fminf(best0, exy + ez0 * ez0);
+            best1 = fminf(best1, exy + ez1 * ez1);
         }
     }
-    if (x >= dx || y >= dy || z >= dz) return;
-    const float d = sqrtf(best);  // sqrt is monotone: the minimum of the distances
-    const float v = 1.0f - fminf(d, trunc) / trunc;
-    if (v != 0.0f) tdf[((long)z * dy + y) * dx + x] = v;  // 0 is already there
+    if (x >= dx || y >= dy) return;
+    // sqrt is monotone: the minimum of the distances
+    const float v0 = 1.0f - fminf(sqrtf(best0), trunc) / trunc;
+    const float v1 = 1.0f - fminf(sqrtf(best1), trunc) / trunc;
+    float *const o0 = tdf + ((long)z * dy + y) * dx + x, *const o1 = o0 + 4L * dy * dx;
+    if (total <= kAccSeg) {  // the tile's only segment: 0 is already there
+        if (z < dz && v0 != 0.0f) *o0 = v0;
+        if (z + 4 < dz && v1 != 0.0f) *o1 = v1;
+    } else {
+        // Segments meet in the grid by the largest value.  The value is a
+        // non-increasing function of d^2 (each step is correctly rounded,
+        // hence monotone), so the largest segment value is the value of the
+        // smallest d^2; values lie in [0, 1], where the order of the float
+        // bits as integers is the order of the floats.
+        if (z < dz && v0 != 0.0f) atomicMax(reinterpret_cast<int *>(o0), __float_as_int(v0));
+        if (z + 4 < dz && v1 != 0.0f) atomicMax(reinterpret_cast<int *>(o1), __float_as_int(v1));
+    }
 }
 
-__global__ __launch_bounds__(512) void acc_tdf_kernel(const float4 *__restrict__ binned,
-                                                      const int *__restrict__ offsets, int dx,
-                                                      int dy, int dz, int tx, int ty, int tz,
-                                                      int reach, float ox, float oy, float oz,
-                                                      float vs, float trunc,
-                                                      const int *__restrict__ near_count,
-                                                      const int *__restrict__ near_list,
-                                                      float *__restrict__ tdf) {
-    __shared__ float4 sp[512];
+__global__ __launch_bounds__(kAccThreads) void acc_tdf_kernel(
+    const float4 *__restrict__ binned, const int *__restrict__ offsets, int dx, int dy, int dz,
+    int tx, int ty, int tz, int reach, float ox, float oy, float oz, float vs, float trunc,
+    const int *__restrict__ item_count, int *__restrict__ next_item,
+    const int2 *__restrict__ items, float *__restrict__ tdf) {
+    __shared__ float4 sp[kAccThreads];
+    __shared__ float4 wsp[kAccThreads / 64 * kAccThreads];
     __shared__ int bstart[kAccMaxBins];
     __shared__ int bpre[kAccMaxBins + 1];
-    __shared__ int wave_n[8];
-    // workgroup b takes entry b of the list of tiles with a point in reach;
-    // the workgroups past its end exit at once (their tiles keep the zeros
-    // of the fill).  One workgroup per tile leaves the balancing of uneven
-    // tiles to the hardware dispatcher.
-    if ((int)blockIdx.x >= *near_count) return;
-    acc_tile(binned, offsets, near_list[blockIdx.x], dx, dy, dz, tx, ty, tz, reach, ox, oy, oz, vs,
-             trunc, sp, bstart, bpre, wave_n, tdf);
+    __shared__ int wave_n[kAccThreads / 64];
+    __shared__ int item;
+    // A resident grid takes the work items one at a time from a counter, so
+    // heavy and light items balance as they come; every workgroup leaves
+    // once the counter passes the (fixed) item count.
+    const int n = *item_count;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(next_item, 1);
+        __syncthreads();
+        const int it = item;
+        if (it >= n) break;  // uniform
+        const int2 e = items[it];
+        acc_tile(binned, offsets, e.x, e.y, dx, dy, dz, tx, ty, tz, reach, ox, oy, oz, vs, trunc,
+                 sp, wsp, bstart, bpre, wave_n, tdf);
+        __syncthreads();  // the LDS tables and `item` are reused
+    }
 }
 
 int check_dims(int dx, int dy, int dz) {
@@ -1386,21 +1463,27 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     // points binned up to this many voxels outside the grid (>= reach_vox)
     const int margin = reach * kAT;
     const long nb = (nt + 1023) / 1024;  // scan blocks
-    const size_t bytes = sizeof(int) * (3 * (size_t)nt + 2 + nb) + 32 + sizeof(float4) * (size_t)num_pts;
+    // work items: at most one per tile plus one per kAccSeg candidates, and
+    // a point is a candidate of at most (2 reach + 1)^3 tiles
+    const long side = 2 * reach + 1;
+    const size_t max_items = (size_t)nt + (size_t)(side * side * side) * (size_t)num_pts / kAccSeg + 1;
+    const size_t bytes = sizeof(int) * (2 * (size_t)nt + 3 + nb) + 32 +
+                         sizeof(float4) * (size_t)num_pts + sizeof(int2) * max_items;
     void *area = nullptr;
     TDM_TRY(tdm::ctx_scratch(ctx, 0, bytes, &area, s));
     int *counts = static_cast<int *>(area);
-    int *near_count = counts + nt;  // 1 entry, zeroed with the counts
-    int *offsets = near_count + 1;  // nt + 1 entries
-    int *bsum = offsets + nt + 1;  // nb entries
+    int *item_count = counts + nt;  // 1 entry, zeroed with the counts
+    int *next_item = item_count + 1;  // 1 entry, zeroed with the counts
+    int *offsets = next_item + 1;     // nt + 1 entries
+    int *bsum = offsets + nt + 1;     // nb entries
     float4 *binned = reinterpret_cast<float4 *>(
         (reinterpret_cast<uintptr_t>(bsum + nb) + 15) & ~uintptr_t(15));
-    int *near_list = reinterpret_cast<int *>(binned + num_pts);  // nt entries
+    int2 *items = reinterpret_cast<int2 *>(binned + num_pts);  // max_items entries
     const unsigned fb = (unsigned)std::min<long>((nt + 255) / 256, 16384);
     // one launch zeroes the output grid (full lines), the bin counts and the
-    // near-tile count
+    // two work-item counters
     clear_regions(tdf, (size_t)dims[0] * dims[1] * dims[2], reinterpret_cast<unsigned char *>(counts),
-                  sizeof(int) * ((size_t)nt + 1), s);
+                  sizeof(int) * ((size_t)nt + 2), s);
     TDM_LAUNCHED();
     if (num_pts > 0) {
         acc_count_kernel<<<(unsigned)((num_pts + kBinThreads * kBinPts - 1) / (kBinThreads * kBinPts)),
@@ -1424,11 +1507,15 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
     }
     TDM_CHECK(nt < (1L << 31), TDM_ERR_INVALID_ARG, "grid too large");
     acc_near_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(offsets, tx, ty, tz, reach,
-                                                                 near_count, near_list);
+                                                                 item_count, items);
     TDM_LAUNCHED();
-    acc_tdf_kernel<<<(unsigned)nt, 512, 0, s>>>(
+    // six workgroups per CU fit (LDS), on the device's CUs
+    int cus = 0;
+    TDM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const long grid = std::min<long>((long)std::max(cus, 1) * 6, (long)max_items);
+    acc_tdf_kernel<<<(unsigned)grid, kAccThreads, 0, s>>>(
         binned, offsets, dims[0], dims[1], dims[2], tx, ty, tz, reach, origin[0], origin[1],
-        origin[2], voxel_size, trunc_margin, near_count, near_list, tdf);
+        origin[2], voxel_size, trunc_margin, item_count, next_item, items, tdf);
     TDM_LAUNCHED();
     return TDM_OK;
 }

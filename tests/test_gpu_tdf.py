@@ -216,9 +216,10 @@ def test_accurate_tdf_bit_exact(torch_gpu):
 
 @pytest.mark.parametrize("trunc_vox", [5, 12, 30])
 def test_accurate_tdf_dense_and_wide_reach(torch_gpu, trunc_vox):
-    """Dense clusters put thousands of candidates on a tile (several 512-point
-    chunks) and wide truncations walk 5^3 and 9^3 bins: still the oracle's
-    values bit for bit, and voxels out of reach read 0."""
+    """Dense clusters put thousands of candidates on a tile (several work
+    items of 1024 candidates, merged in the grid) and wide truncations walk
+    5^3 and 9^3 bins: still the oracle's values bit for bit, and voxels out of
+    reach read 0."""
     torch = torch_gpu
     rng = np.random.default_rng(100 + trunc_vox)
     vs = np.float32(0.01)
