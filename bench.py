@@ -866,9 +866,10 @@ def main() -> int:
                     help="precision of the headline pipeline (default fp32, the plain path)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving sweep (e.g. under a profiler)")
-    ap.add_argument("--only", nargs="+", choices=["tsdf", "tdf", "net"],
+    ap.add_argument("--only", nargs="+", choices=["tsdf", "tdf", "net", "batched"],
                     help="run only these workloads and print their figures (for counter runs); "
-                         "net: three descriptor passes of --batch keypoints at --precision")
+                         "net: three descriptor passes of --batch keypoints at --precision; "
+                         "batched: the 64-fragment batched form in bf16 (hipGraph)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (reference for HBM counters)")
     args = ap.parse_args()
@@ -909,6 +910,13 @@ def main() -> int:
         if "net" in args.only:
             log(f"descriptor passes, {args.precision}")
             out["net"] = net_passes(torch, args.precision, args.batch)
+        if "batched" in args.only:
+            log("batched form (hipGraph), bf16")
+            tmp = tempfile.mkdtemp(prefix="tdm_bench_")
+            js, wt = os.path.join(tmp, "net.json"), os.path.join(tmp, "w.marvin")
+            netdef.write_net_json(js)
+            tdm.marvin_write(wt, netdef.he_normal_weights(5))
+            out["batched"] = batched_bench(torch, js, wt, replays=1)
         print(json.dumps(out), flush=True)
         return 0
     log("building inputs")
