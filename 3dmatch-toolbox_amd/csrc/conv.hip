@@ -210,7 +210,33 @@ struct ConvArgs {
     int relu;
     const float *colscale;  // fp8: per output channel dequantization scale (else NULL)
     float oscale;           // fp8 outputs: multiplier before conversion (else 1)
+    // Row decode when M < 2^31 (small_rows): n / E = umulhi(n, rdiv_mul) >>
+    // rdiv_sh for n < 2^31, E the extent rows are counted in (OD, or OD/2
+    // with a fused pool; rdiv_mul = 0: E == 1); else 64-bit divisions.
+    unsigned rdiv_mul;
+    int rdiv_sh;
+    int small_rows;
 };
+
+// Multiply-high division by d for n < 2^31 (Granlund and Montgomery's
+// round-up method: l = ceil(log2 d), m = ceil(2^(31 + l) / d) < 2^32, then
+// n / d = (n * m) >> (31 + l); exact because m * d - 2^(31 + l) < d <= 2^l).
+void fast_div(int d, unsigned *mul, int *sh) {
+    if (d == 1) {
+        *mul = 0u;
+        *sh = 0;
+        return;
+    }
+    int l = 0;
+    while ((1L << l) < d) ++l;
+    const unsigned long long num = 1ull << (31 + l);
+    *mul = (unsigned)((num + (unsigned long long)d - 1) / (unsigned long long)d);
+    *sh = l - 1;
+}
+
+__device__ inline unsigned row_div(unsigned n, const ConvArgs &p) {
+    return p.rdiv_mul ? __umulhi(n, p.rdiv_mul) >> p.rdiv_sh : n;
+}
 
 // Workgroup -> tile order that keeps neighbouring tiles on one XCD.  The
 // dispatcher hands workgroup i (linear id, x fastest) to XCD i % 8, so
@@ -233,10 +259,28 @@ __device__ inline void xcd_tile(unsigned &tx, unsigned &ty, unsigned &tz) {
 // Input voxel index of output row m: rows are (b, z, y, x) in order, or, with
 // POOL, (b, pz, py, px, window) with the 2x2x2 window innermost.
 template <bool POOL>
-__device__ inline long row_voxel(long m, int D, int OD) {
+__device__ inline long row_voxel(long m, const ConvArgs &p) {
+    const int D = p.D, OD = p.OD;
     int ox, oy, oz;
     long b;
-    if (POOL) {
+    if (p.small_rows) {
+        // rows fit 31 bits: the three divisions by the extent are multiply-highs
+        const unsigned E = POOL ? (unsigned)(OD >> 1) : (unsigned)OD;
+        const unsigned q0 = POOL ? (unsigned)(m >> 3) : (unsigned)m;
+        const unsigned q1 = row_div(q0, p), q2 = row_div(q1, p), q3 = row_div(q2, p);
+        const int cx = (int)(q0 - q1 * E), cy = (int)(q1 - q2 * E), cz = (int)(q2 - q3 * E);
+        b = q3;
+        if (POOL) {
+            const int w = (int)(m & 7);
+            ox = 2 * cx + (w & 1);
+            oy = 2 * cy + ((w >> 1) & 1);
+            oz = 2 * cz + (w >> 2);
+        } else {
+            ox = cx;
+            oy = cy;
+            oz = cz;
+        }
+    } else if (POOL) {
         const int w = (int)(m & 7);
         long q = m >> 3;
         const int PD = OD >> 1;
@@ -282,7 +326,7 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         long m = min(m0 + 16 * i + r16, p.M - 1);
-        ap[i] = in + row_voxel<POOL>(m, p.D, p.OD) * p.C + lanek;
+        ap[i] = in + row_voxel<POOL>(m, p) * p.C + lanek;
         bp[i] = w + (long)(n0 + 16 * i + r16) * K + lanek;
     }
 
@@ -509,7 +553,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
     for (int j = 0; j < A_PT; ++j) {
         const int idx = tid + 256 * j, row = idx / PCS, pc = idx % PCS;
         const long m = min(mb + row, p.M - 1);
-        a_src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + pc * 16;
+        a_src[j] = in + (row_voxel<POOL>(m, p) * C) * (long)sizeof(T) + pc * 16;
         a_dst[j] = row * kLdsRow + ((pc ^ swz<STAGE>(row)) << 4);
     }
     const unsigned char *b_src[B_PT];
@@ -686,7 +730,7 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
         isa[j] = row < BM;
         if (row < BM) {
             const long m = min(mb + row, p.M - 1);
-            src[j] = in + (row_voxel<POOL>(m, D, p.OD) * C) * (long)sizeof(T) + ps * 16;
+            src[j] = in + (row_voxel<POOL>(m, p) * C) * (long)sizeof(T) + ps * 16;
         } else {
             src[j] = w + ((long)(nb + row - BM) * K) * (long)sizeof(T) + ps * 16;
         }
@@ -818,22 +862,6 @@ struct Conv1Args {
     unsigned od_mul;
     int od_sh;
 };
-
-// Multiply-high division by d for n < 2^31 (Granlund and Montgomery's
-// round-up method: l = ceil(log2 d), m = ceil(2^(31 + l) / d) < 2^32, then
-// n / d = (n * m) >> (31 + l); exact because m * d - 2^(31 + l) < d <= 2^l).
-void fast_div(int d, unsigned *mul, int *sh) {
-    if (d == 1) {
-        *mul = 0u;
-        *sh = 0;
-        return;
-    }
-    int l = 0;
-    while ((1L << l) < d) ++l;
-    const unsigned long long num = 1ull << (31 + l);
-    *mul = (unsigned)((num + (unsigned long long)d - 1) / (unsigned long long)d);
-    *sh = l - 1;
-}
 
 __device__ inline unsigned div_od(unsigned n, const Conv1Args &a) {
     return a.od_mul ? __umulhi(n, a.od_mul) >> a.od_sh : n;
@@ -1113,6 +1141,8 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     p.splits = splits;
     p.relu = d.relu ? 1 : 0;
     p.M = (long)batch * d.OD * d.OD * d.OD;
+    p.small_rows = p.M < (1L << 31) ? 1 : 0;
+    fast_div(d.pool ? d.OD / 2 : d.OD, &p.rdiv_mul, &p.rdiv_sh);
     const bool lds = tdm::conv_uses_lds(dtype_of<T>(), d);
     if (lds) {
         int v = variant;
