@@ -700,7 +700,8 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
     constexpr int SK = STAGE / (int)sizeof(T);
     constexpr int SLOTS = (BM + BN) * PCS;  // 16-byte slots per stage
     constexpr int J = SLOTS / 256;          // load instructions per wave per stage
-    static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0, "stage layout");
+    static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0 && (BM * PCS) % 64 == 0,
+                  "stage layout");
     static_assert(NST == 2 || NST == 3, "ring depth");
     static_assert((NST - 2) * J < 64, "vmcnt range");
     constexpr int SB = (BM + BN) * STAGE;
@@ -708,7 +709,10 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
     __shared__ __attribute__((aligned(16))) unsigned char L1[SB];
     __shared__ __attribute__((aligned(16))) unsigned char L2[NST > 2 ? SB : 16];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: read it into a scalar so that everything
+    // derived from it (LDS slots, A or B role of a load) stays scalar
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / WNW, wn = wave % WNW;
     unsigned tx, ty, tz;
     xcd_tile(tx, ty, tz);
@@ -716,23 +720,33 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
     const int nb = ty * BN;
     const int C = p.C, D = p.D, ks = p.ksize;
     const long K = (long)p.taps * C;
-    const unsigned char *__restrict__ in = static_cast<const unsigned char *>(p.in);
-    const unsigned char *__restrict__ w = static_cast<const unsigned char *>(p.w);
+    // Scalar bases: the input voxel of the tile's first row (the smallest of
+    // its rows: row_voxel grows with m, and the first row of a pooled tile
+    // is window element 0) and the tile's first weight row; each lane keeps
+    // a 32-bit byte offset from them (a tile spans a few patches of input
+    // and at most N*K weights, far below 4 GB).
+    const long vb0 = row_voxel<POOL>(mb, p);
+    const unsigned char *__restrict__ a_base =
+        static_cast<const unsigned char *>(p.in) + vb0 * C * (long)sizeof(T);
+    const unsigned char *__restrict__ b_base =
+        static_cast<const unsigned char *>(p.w) + (long)nb * K * (long)sizeof(T);
 
-    // load j of this wave fills slots (4j + wave)*64 + lane of a stage buffer
-    const unsigned char *src[J];
+    // load j of this wave fills slots (4j + wave)*64 + lane of a stage
+    // buffer; the 64 slots of one load lie in A rows or all in B rows (BM *
+    // PCS is a multiple of 64), so the role is uniform
+    unsigned voff[J];
     bool isa[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int slot = (4 * j + wave) * 64 + lane;
         const int row = slot / PCS, pl = slot % PCS;
         const int ps = pl ^ swz<STAGE>(row);
-        isa[j] = row < BM;
-        if (row < BM) {
+        isa[j] = (4 * j + wave) * 64 < BM * PCS;
+        if (isa[j]) {
             const long m = min(mb + row, p.M - 1);
-            src[j] = in + (row_voxel<POOL>(m, p) * C) * (long)sizeof(T) + ps * 16;
+            voff[j] = (unsigned)((row_voxel<POOL>(m, p) - vb0) * C * (long)sizeof(T)) + ps * 16;
         } else {
-            src[j] = w + ((long)(nb + row - BM) * K) * (long)sizeof(T) + ps * 16;
+            voff[j] = (unsigned)((long)(row - BM) * K * (long)sizeof(T)) + ps * 16;
         }
     }
     const int r16 = lane & 15, g = lane >> 4;
@@ -771,10 +785,11 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
         }
         const long aoff = ((long)((kz * D + ky) * D + kx) * C + fc0) * (long)sizeof(T);
         const long boff = ((long)ftap * C + fc0) * (long)sizeof(T);
+        const unsigned char *const sa = a_base + aoff, *const sb = b_base + boff;
 #pragma unroll
         for (int j = 0; j < J; ++j)
             __builtin_amdgcn_global_load_lds(
-                (const void *)(src[j] + (isa[j] ? aoff : boff)),
+                (const void *)((isa[j] ? sa : sb) + voff[j]),
                 (__attribute__((address_space(3))) void *)(buf + (4 * j + wave) * 1024), 16, 0, 0);
         fc0 += SK;
         if (fc0 == C) {
