@@ -4,8 +4,9 @@
 // stride 1, no padding) is an implicit GEMM over NDHWC activations:
 //   rows  m = output voxels (patch, z, y, x)
 //   cols  n = output channels
-//   k     = (tap, input channel), tap-major, so for a fixed tap the k-run of
-//           a row is the contiguous channel vector of one input voxel.
+//   k     = (tap, input channel), laid out tap-major, so for a fixed tap the
+//           k-run of a row is the contiguous channel vector of one input
+//           voxel (the staged kernels sum it in the order of KWalk).
 // Weights are packed [n][tap][c], so both operands are k-contiguous and a lane
 // loads its MFMA fragment with one 16-byte load:
 //   fp32  v_mfma_f32_16x16x4_f32: a 16-byte load holds 4 consecutive k; the
@@ -514,6 +515,50 @@ __device__ inline int swz(int r) {
     return STAGE == 64 ? ((r >> 1) ^ (r >> 3)) & 3 : ((r >> 1) ^ (r << 2)) & 7;
 }
 
+// The order of the reduction, shared by every staged variant so that all of
+// them sum each output's k in the same order.  k runs over units of 128 bytes
+// of input channels (64 bytes when the channel vector is not a multiple of
+// 128 bytes), the unit outermost, then the tap, then the stage's half of the
+// unit: a tile reads one channel unit of its input rows for all 27 taps
+// before the next, so the tiles in flight on an XCD touch 128 bytes of each
+// input voxel at a time rather than the whole channel vector, and their
+// working set stays within the XCD's 4 MB L2 (with the whole vector, tap
+// after tap, fp32 layers re-read their input from beyond L2 ~17 times over).
+// Split-K boundaries fall on unit edges.
+struct KWalk {
+    int taps = 1, ub = 128, half_n = 1, tap = 0, unit = 0, half = 0, csz = 4, stage = 128;
+    // returns the number of stages
+    __device__ int init(int C, int esz, int STAGE, int ntaps) {
+        taps = ntaps;
+        csz = esz;
+        stage = STAGE;
+        ub = (C * esz) % 128 == 0 ? 128 : 64;
+        half_n = ub / STAGE;
+        return ntaps * (C * esz / ub) * half_n;
+    }
+    __device__ void split(int ns, unsigned tz, int splits, int *s0, int *s1) const {
+        const int nc = ns / half_n;  // units
+        *s0 = (int)((long)nc * tz / splits) * half_n;
+        *s1 = tz + 1 == (unsigned)splits ? ns : (int)((long)nc * (tz + 1) / splits) * half_n;
+    }
+    __device__ void seek(int s) {
+        const int u = s / half_n;
+        half = s - u * half_n;
+        unit = u / taps;
+        tap = u - unit * taps;
+    }
+    __device__ int c0() const { return (unit * ub + half * stage) / csz; }
+    __device__ void next() {
+        if (++half == half_n) {
+            half = 0;
+            if (++tap == taps) {
+                tap = 0;
+                ++unit;
+            }
+        }
+    }
+};
+
 template <int WMW, int WTM, int WTN, int STAGE = 128>
 struct Tile {
     static constexpr int WNW = 4 / WMW;
@@ -531,7 +576,6 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
     constexpr int PCS = STAGE / 16;                   // 16-byte pieces per staged row
     constexpr int HALVES = STAGE / 64;
     constexpr int IB = WTM / 16, JB = WTN / 16;
-    constexpr int SK = STAGE / (int)sizeof(T);        // k per stage
     constexpr int A_PT = BM * PCS / 256;              // 16-byte pieces per thread
     constexpr int B_PT = BN * PCS / 256;
     __shared__ __attribute__((aligned(16))) unsigned char lds[2][(BM + BN) * kLdsRow];
@@ -565,26 +609,19 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         b_dst[j] = (BM + col) * kLdsRow + ((pc ^ swz<STAGE>(BM + col)) << 4);
     }
 
-    const int cps = C / SK;  // stages per tap
-    const int ns = p.taps * cps;
+    KWalk kw;
+    const int ns = kw.init(C, (int)sizeof(T), STAGE, p.taps);
     int s0 = 0, s1 = ns;
-    if (SPLIT) {
-        // split points in 128-byte units of k, whatever the stage size, so
-        // every variant sums the same k ranges into each partial
-        constexpr int CG = 128 / STAGE;
-        const int nc = ns / CG;
-        s0 = (int)((long)nc * tz / p.splits) * CG;
-        s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
-    }
+    if (SPLIT) kw.split(ns, tz, p.splits, &s0, &s1);
     // Stages are fetched strictly in order s0, s0+1, ...: the tap and channel
     // offset advance with each fetch instead of being divided out of s.
-    int ftap = s0 / cps, fc0 = (s0 - ftap * cps) * SK;
+    kw.seek(s0);
     // Two register sets: the loads of stage s+2 are issued before the MFMAs
     // of stage s, and stage s+1 (loaded one stage earlier) goes to LDS after
     // them, so a global load has two stages of compute to arrive.
     uint4 ra0[A_PT], rb0[B_PT], ra1[A_PT], rb1[B_PT];
     auto fetch = [&](uint4 (&ra)[A_PT], uint4 (&rb)[B_PT]) {
-        const int tap = ftap, c0 = fc0;
+        const int tap = kw.tap, c0 = kw.c0();
         int kz = 0, ky = 0, kx = 0;
         if (ks == 3) {
             kz = tap / 9;
@@ -597,11 +634,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
         for (int j = 0; j < A_PT; ++j) ra[j] = ld16(a_src[j] + aoff);
 #pragma unroll
         for (int j = 0; j < B_PT; ++j) rb[j] = ld16(b_src[j] + boff);
-        fc0 += SK;
-        if (fc0 == C) {
-            fc0 = 0;
-            ++ftap;
-        }
+        kw.next();
     };
     auto stash = [&](int buf, const uint4 (&ra)[A_PT], const uint4 (&rb)[B_PT]) {
 #pragma unroll
@@ -697,7 +730,6 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
     constexpr int BM = TL::BM, BN = TL::BN, WNW = TL::WNW;
     constexpr int PCS = STAGE / 16, HALVES = STAGE / 64;
     constexpr int IB = WTM / 16, JB = WTN / 16;
-    constexpr int SK = STAGE / (int)sizeof(T);
     constexpr int SLOTS = (BM + BN) * PCS;  // 16-byte slots per stage
     constexpr int J = SLOTS / 256;          // load instructions per wave per stage
     static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0 && (BM * PCS) % 64 == 0,
@@ -766,17 +798,13 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
             b_off[j][h] = row * STAGE + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
     }
 
-    const int cps = C / SK;
-    const int ns = p.taps * cps;
+    KWalk kw;  // the next stage to copy
+    const int ns = kw.init(C, (int)sizeof(T), STAGE, p.taps);
     int s0 = 0, s1 = ns;
-    if (SPLIT) {
-        constexpr int CG = 128 / STAGE;
-        const int nc = ns / CG;
-        s0 = (int)((long)nc * tz / p.splits) * CG;
-        s1 = tz + 1 == (unsigned)p.splits ? ns : (int)((long)nc * (tz + 1) / p.splits) * CG;
-    }
-    int ftap = s0 / cps, fc0 = (s0 - ftap * cps) * SK;  // the next stage to copy
+    if (SPLIT) kw.split(ns, tz, p.splits, &s0, &s1);
+    kw.seek(s0);
     auto issue = [&](unsigned char *buf) {
+        const int ftap = kw.tap, fc0 = kw.c0();
         int kz = 0, ky = 0, kx = 0;
         if (ks == 3) {
             kz = ftap / 9;
@@ -791,11 +819,7 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
             __builtin_amdgcn_global_load_lds(
                 (const void *)((isa[j] ? sa : sb) + voff[j]),
                 (__attribute__((address_space(3))) void *)(buf + (4 * j + wave) * 1024), 16, 0, 0);
-        fc0 += SK;
-        if (fc0 == C) {
-            fc0 = 0;
-            ++ftap;
-        }
+        kw.next();
     };
 
     floatx4 acc[IB][JB];
@@ -1257,8 +1281,11 @@ bool conv_uses_lds(DType dt, const ConvDesc &d) {
 // Reduction steps in 128-byte units of k (the unit split-K divides).
 int conv_reduction_steps(DType dt, const ConvDesc &d) {
     const int taps = d.ksize * d.ksize * d.ksize;
-    if (conv_uses_lds(dt, d))
-        return std::max(1, taps * d.C / (dt == DType::F32 ? 32 : dt == DType::F8 ? 128 : 64));
+    if (conv_uses_lds(dt, d)) {
+        // units of the staged kernels' reduction order (KWalk)
+        const int cb = d.C * (int)dtype_size(dt);
+        return std::max(1, taps * (cb / (cb % 128 == 0 ? 128 : 64)));
+    }
     return taps * d.C / (dt == DType::F32 ? 16 : 32);
 }
 
