@@ -15,7 +15,8 @@ inline size_t dtype_size(DType t) { return t == DType::F32 ? 4 : t == DType::F8 
 
 // One 3-D convolution layer in implicit-GEMM form over NDHWC activations:
 // rows = output voxels of `batch` cubes, cols = output channels, reduction =
-// taps x input channels (tap-major, channel-minor; weights packed [N][taps][C]).
+// taps x input channels (laid out tap-major, channel-minor: weights packed
+// [N][taps][C]; the staged kernels sum 128-byte channel units outermost).
 struct ConvDesc {
     int C = 0;      // input channels (multiple of 32)
     int N = 0;      // output channels (multiple of 64)
