@@ -31,12 +31,21 @@ SPECS = [(20_000, 0, 0), (0, 0, 0), (55_000, 7, 0), (9_000, 13, 40), (31_000, 29
 
 @pytest.mark.parametrize("decode", [tdm.DECODE_EXACT, tdm.DECODE_REFERENCE])
 def test_batch_tdf_equals_per_cloud(torch_gpu, decode):
+    """Every grid of the plan equals the oracle's occupancy + ComputeTDF of
+    its cloud bit for bit (points outside a shrunk grid dropped, an empty
+    cloud all zeros), and the per-cloud entry point gives the same bits."""
     torch = torch_gpu
     clouds = _clouds(torch, SPECS)
     plan = tdm.TdfPlan(clouds, 0.01, 0.05, decode)
     plan.run()
     torch.cuda.synchronize()
+    mode = "exact" if decode == tdm.DECODE_EXACT else "reference"
     for pts, tdf, dims, origin in clouds:
+        print(f"case plan cloud dims={dims} decode={mode}", flush=True)
+        occ = oracle.occupancy(pts.cpu().numpy(), dims, origin, np.float32(0.01))
+        want = oracle.compute_tdf(occ, np.float32(0.01), np.float32(0.05), mode)
+        got = tdf.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), dims
         ref = torch.empty_like(tdf)
         tdm.tdf_from_points(pts, dims, origin, np.float32(0.01), np.float32(0.05), ref, decode)
         torch.cuda.synchronize()
