@@ -1303,7 +1303,19 @@ bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     if (v < 0 || v >= kConvVariants) return false;
     const int bn = (4 / kVariants[v][0]) * kVariants[v][2];
     const int stage_k = kVariants[v][3] / (int)dtype_size(dt);
-    return d.N % bn == 0 && d.C % stage_k == 0;
+    if (d.N % bn != 0 || d.C % stage_k != 0) return false;
+    if (v >= 9) {
+        // conv_dma_kernel addresses a tile's input rows and weight rows with
+        // 32-bit byte offsets from scalar bases: the input of the cubes a
+        // tile's rows can span, and the weights, must stay below 4 GB
+        const long bm = (long)kVariants[v][0] * kVariants[v][1];
+        const long ocube = (long)d.OD * d.OD * d.OD;
+        const long icube = (long)d.D * d.D * d.D * d.C * (long)dtype_size(dt);
+        const long K = (long)d.ksize * d.ksize * d.ksize * d.C;
+        if ((bm / ocube + 2) * icube >= (1L << 32)) return false;
+        if ((long)d.N * K * (long)dtype_size(dt) >= (1L << 32)) return false;
+    }
+    return true;
 }
 
 int conv_default_variant(DType dt, const ConvDesc &d) {
