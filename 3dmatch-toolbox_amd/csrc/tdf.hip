@@ -613,30 +613,16 @@ __device__ inline bool bit_at(const unsigned *__restrict__ bits, long rw, int dy
     return (bits[((long)z * dy + y) * rw + (x >> 5)] >> (x & 31)) & 1u;
 }
 
-// Reference decode: voxels whose float decode differs from the true
-// coordinates search around the decoded coordinates instead (core/demo.cu:
-// 44-56 with the clipped bounds), occupancy still taken at the true index.
-__global__ void tdf_refdecode_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
-                                     int r, Lut lut, float *__restrict__ tdf) {
-    const long n = (long)dx * dy * dz;
-    const long rw = row_words(dx);
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += (long)gridDim.x * blockDim.x) {
-        int x, y, z;
-        float_decode(v, dx, dy, x, y, z);
-        const int tx = (int)(v % dx), ty = (int)((v / dx) % dy), tz = (int)(v / ((long)dx * dy));
-        if (x == tx && y == ty && z == tz) continue;
-        if (bit_at(bits, rw, dy, tx, ty, tz)) continue;  // occupied: 1, already set
-        int best = kLutSize;
-        for (int ix = max(0, x - r); ix < min(dx, x + r + 1); ++ix)
-            for (int iy = max(0, y - r); iy < min(dy, y + r + 1); ++iy)
-                for (int iz = max(0, z - r); iz < min(dz, z + r + 1); ++iz)
-                    if (bit_at(bits, rw, dy, ix, iy, iz)) {
-                        int d2 = (x - ix) * (x - ix) + (y - iy) * (y - iy) + (z - iz) * (z - iz);
-                        best = min(best, d2);
-                    }
-        tdf[v] = best < kLutSize ? lut.v[best] : 0.0f;
-    }
+// n floats from src to dst: 16-byte pieces over a grid-stride loop, then the
+// tail (n need not be a multiple of 4; both buffers come from hipMalloc).
+__global__ void copy_floats_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                   long n) {
+    const long n4 = n / 4;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        reinterpret_cast<float4 *>(dst)[i] = reinterpret_cast<const float4 *>(src)[i];
+    const long t = 4 * n4 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
 }
 
 // Reference decode for one grid, from the exact TDF in a separate buffer.
@@ -645,7 +631,7 @@ __global__ void tdf_refdecode_kernel(const unsigned *__restrict__ bits, int dx, 
 // is exact[v]; where it is another voxel j inside the grid the search is the
 // one that gives j its exact TDF (an occupied j is within reach at distance
 // 0), so v gets 1 if occupied, else exact[j]; decodes outside the grid are
-// searched here as in tdf_refdecode_kernel.
+// searched here around the decoded voxel (core/demo.cu:44-56, bounds clipped).
 __global__ void tdf_refgather_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
                                      int r, Lut lut, const float *__restrict__ exact,
                                      float *__restrict__ tdf) {
@@ -1533,6 +1519,9 @@ struct tdm_tdf_plan {
     unsigned char *flags = nullptr;
     long total_vox = 0;
     std::vector<CloudDev> host;  // the same table on the host
+    // reference decode: the exact TDF of one cloud above 2^24 voxels at a
+    // time (the largest such cloud's size), gathered from by the decode
+    float *ref_exact = nullptr;
 };
 
 int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, float voxel_size,
@@ -1600,6 +1589,21 @@ int tdm_tdf_plan_create(tdm_ctx *ctx, const tdm_cloud_ref *clouds, int num, floa
                          "TDF plan of %zu bytes: %s", bit_bytes, hipGetErrorString(e));
     }
     plan->flags = reinterpret_cast<unsigned char *>(plan->bits + words);
+    if (decode == TDM_DECODE_REFERENCE) {
+        long big = 0;  // voxels of the largest cloud whose float32 decode can differ
+        for (const CloudDev &d : plan->host) {
+            const long n = (long)d.dims[0] * d.dims[1] * d.dims[2];
+            if (n > (1L << 24)) big = std::max(big, n);
+        }
+        if (big > 0 && (e = hipMalloc((void **)&plan->ref_exact, sizeof(float) * big)) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(plan->table);
+            (void)hipFree(plan->bits);
+            return tdm::fail(e == hipErrorOutOfMemory ? TDM_ERR_NOMEM : TDM_ERR_HIP,
+                             "TDF plan: %ld voxels of reference-decode scratch: %s", big,
+                             hipGetErrorString(e));
+        }
+    }
     *out = plan.release();
     return TDM_OK;
 }
@@ -1642,12 +1646,23 @@ int tdm_tdf_plan_run(tdm_tdf_plan *plan, void *stream) {
     default: return tdm::fail(TDM_ERR_UNSUPPORTED, "radius %d", plan->r);
     }
     TDM_LAUNCHED();
+    // Reference decode.  Up to 2^24 voxels the float32 decode of every index
+    // is exact (all its operands are integers below 2^24, and the quotient
+    // v / (dx*dy) = k - r/(dx*dy) lies more than half an ulp below the next
+    // integer k), so those grids are done.  A larger grid's exact TDF is
+    // copied to the plan's scratch and each voxel takes the value at its
+    // decoded index from there, as the single-cloud path does.
     if (plan->decode == TDM_DECODE_REFERENCE)
         for (const CloudDev &d : plan->host) {
-            const long n = (long)d.dims[0] * d.dims[1] * d.dims[2];
-            const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
-            tdf_refdecode_kernel<<<blocks, 256, 0, s>>>(plan->bits + d.word_off, d.dims[0],
-                                                         d.dims[1], d.dims[2], plan->r, lut, d.tdf);
+            const int dx = d.dims[0], dy = d.dims[1], dz = d.dims[2];
+            const long n = (long)dx * dy * dz;
+            if (n <= (1L << 24)) continue;
+            const unsigned cb = (unsigned)std::min<long>((n / 4 + 255) / 256, 16384);
+            copy_floats_kernel<<<cb, 256, 0, s>>>(d.tdf, plan->ref_exact, n);
+            TDM_LAUNCHED();
+            const unsigned blocks = (unsigned)((long)((dx + 255) / 256) * dy * dz);
+            tdf_refgather_kernel<<<blocks, 256, 0, s>>>(plan->bits + d.word_off, dx, dy, dz,
+                                                         plan->r, lut, plan->ref_exact, d.tdf);
             TDM_LAUNCHED();
         }
     return TDM_OK;
@@ -1660,6 +1675,7 @@ int tdm_tdf_plan_destroy(tdm_tdf_plan *plan) {
     (void)hipDeviceSynchronize();
     if (plan->table) (void)hipFree(plan->table);
     if (plan->bits) (void)hipFree(plan->bits);
+    if (plan->ref_exact) (void)hipFree(plan->ref_exact);
     delete plan;
     return TDM_OK;
 }

@@ -761,11 +761,21 @@ def batched_bench(torch, js, wt, n_frag=64, seed=40, replays=2, prec="bf16"):
     tdf_ms = e0.elapsed_time(e1)
     voxels = sum(d[0] * d[1] * d[2] for _, _, d, _ in clouds)
     plan.close()
+    # the same plan with the reference's float32 voxel decode
+    plan = tdm.TdfPlan(clouds, 0.01, 0.05, tdm.DECODE_REFERENCE)
+    plan.run()
+    e0.record()
+    plan.run()
+    e1.record()
+    torch.cuda.synchronize()
+    tdf_ref_ms = e0.elapsed_time(e1)
+    plan.close()
     net.close()
     return {"fragments": n_frag, "points": total_pts, "keypoints": n_kp, "voxels": voxels,
             "eager_s": eager, "graph_s": graph,
             "descriptors_per_s_graph": n_kp / graph, "descriptors_per_s_eager": n_kp / eager,
             "graph_equals_eager": same, "tdf_plan_ms": tdf_ms,
+            "tdf_plan_reference_decode_ms": tdf_ref_ms,
             "tdf_gvoxels_per_s": voxels / (tdf_ms / 1e3) / 1e9, "tile_variants": variants,
             "config": ("64 fragments, points U[20k,500k], extents U[1,4] m, keypoints "
                        f"U[1000,10000], {prec} net, TDF plan + multi-grid pass in one hipGraph")}

@@ -222,8 +222,10 @@ int tdm_tdf_accurate(tdm_ctx *ctx, const float *pts, int num_pts, const int dims
  * batch of fragments of different sizes, as a plan (made once, on the host)
  * and runs (four launches for the whole batch: zero every cloud's output
  * grid, clear the bit grids, scatter every cloud's points, one tile kernel
- * over every grid's tiles; in reference decode one more launch per cloud
- * rewrites the voxels whose float32 decode differs).  A run
+ * over every grid's tiles; in reference decode two more per cloud above 2^24
+ * voxels -- the only grids whose float32 decode differs -- copy its exact TDF
+ * to plan-owned scratch, allocated at plan creation, and gather each voxel's
+ * value at its decoded index).  A run
  * allocates nothing, sets no memory with memset and never waits on the host,
  * so it can be captured into a hipGraph and replayed; it reads the point
  * buffers named in the plan at run time.  Results are bit-identical to

@@ -87,6 +87,26 @@ def test_float_decode_counts():
             assert oracle.float_decode(int(i), dims[0], dims[1]) == (x[i], y[i], z[i])
 
 
+def test_float_decode_exact_up_to_2_24_voxels():
+    """The batched plan skips the reference-decode pass on grids of at most
+    2^24 voxels: there the float32 decode of every index equals the integer
+    one.  Shapes at the limit with one huge, one flat and one cubic slice,
+    and a grid just above the limit where the decode does move voxels."""
+    for dims in ((4095, 4097, 1), (1, 1, 1 << 24), (255, 257, 256), (1024, 1024, 16)):
+        n = dims[0] * dims[1] * dims[2]
+        assert n <= 1 << 24
+        x, y, z = oracle.float_decode_all(n, dims[0], dims[1])
+        v = np.arange(n)
+        assert np.array_equal(x, v % dims[0]), dims
+        assert np.array_equal(y, (v // dims[0]) % dims[1]), dims
+        assert np.array_equal(z, v // (dims[0] * dims[1])), dims
+    dims = (419, 321, 125)  # 16.8 M voxels, just above 2^24
+    n = dims[0] * dims[1] * dims[2]
+    x, y, z = oracle.float_decode_all(n, dims[0], dims[1])
+    v = np.arange(n)
+    assert np.any((x != v % dims[0]) | (z != v // (dims[0] * dims[1])))
+
+
 def test_occupancy_and_keypoint_rounding():
     cloud = synth.fragment(5000, seed=6)
     vs = np.float32(0.01)
