@@ -53,6 +53,35 @@ def test_batch_tdf_equals_per_cloud(torch_gpu, decode):
     plan.close()
 
 
+def test_batch_reference_decode_two_large_grids(torch_gpu):
+    """Reference decode in a plan with two grids above 2^24 voxels (each
+    gathers from its exact TDF through the same plan-owned scratch, one
+    after the other) and a small one between them (no pass needed): every
+    grid equals the oracle's float32-decode ComputeTDF bit for bit."""
+    torch = torch_gpu
+    vs, tr = np.float32(0.01), np.float32(0.05)
+    clouds, hosts = [], []
+    # views of a 4 m cube (grids of 23 M and 19 M voxels) around a 1 m one
+    for seed, n, extent in ((900, 20_000, 4.0), (1900, 8_000, 1.0), (903, 20_000, 4.0)):
+        cloud = synth.cropped_fragment(seed, n, extent)
+        dims, origin = oracle.grid_params(cloud, vs, 15)
+        pts = torch.from_numpy(np.ascontiguousarray(cloud)).to("cuda:0")
+        clouds.append((pts, torch.full((dims[2], dims[1], dims[0]), -1.0, device="cuda:0"), dims,
+                       origin))
+        hosts.append(cloud)
+    nvox = [d[0] * d[1] * d[2] for _, _, d, _ in clouds]
+    print(f"case plan reference decode, grids {nvox}", flush=True)
+    assert nvox[0] > 1 << 24 and nvox[2] > 1 << 24 and nvox[1] <= 1 << 24
+    plan = tdm.TdfPlan(clouds, vs, tr, tdm.DECODE_REFERENCE)
+    plan.run()
+    torch.cuda.synchronize()
+    for cloud, (pts, tdf, dims, origin) in zip(hosts, clouds):
+        occ = oracle.occupancy(cloud, dims, origin, vs)
+        want = oracle.compute_tdf(occ, vs, tr, "reference")
+        assert np.array_equal(tdf.cpu().numpy().view(np.uint32), want.view(np.uint32)), dims
+    plan.close()
+
+
 def test_batch_graph_replay(torch_gpu, tmp_path):
     torch = torch_gpu
     js, wt = str(tmp_path / "n.json"), str(tmp_path / "w.marvin")
