@@ -217,6 +217,9 @@ struct ConvArgs {
     unsigned rdiv_mul;
     int rdiv_sh;
     int small_rows;
+    // the same for E + 1 (the pooled group kernel's slots per window line)
+    unsigned sdiv_mul;
+    int sdiv_sh;
 };
 
 // Multiply-high division by d for n < 2^31 (Granlund and Montgomery's
@@ -237,6 +240,9 @@ void fast_div(int d, unsigned *mul, int *sh) {
 
 __device__ inline unsigned row_div(unsigned n, const ConvArgs &p) {
     return p.rdiv_mul ? __umulhi(n, p.rdiv_mul) >> p.rdiv_sh : n;
+}
+__device__ inline unsigned slot_div(unsigned n, const ConvArgs &p) {
+    return __umulhi(n, p.sdiv_mul) >> p.sdiv_sh;
 }
 
 // Workgroup -> tile order that keeps neighbouring tiles on one XCD.  The
@@ -436,12 +442,52 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
 
 // Epilogue of a wave tile of IB x JB 16x16 blocks at (m0, n0): the lane holds
 // rows m0+16i+4g+r (r = 0..3) of column n0+16j+r16.  Split-K writes float
-// partials; otherwise bias, ReLU, the fused 2x2x2 pool and the store.
-template <typename OT, int IB, int JB, bool POOL, bool SPLIT>
+// partials; otherwise bias (after the fp8 column scale when SCALED), ReLU,
+// the fused 2x2x2 pool and the store.  The pool takes the window's largest
+// accumulator first and applies scale, bias and ReLU to it once: each of
+// those is monotone non-decreasing (rounding included, the scales are
+// non-negative), so this equals the largest of the transformed values bit
+// for bit.
+template <typename OT, int IB, int JB, bool POOL, bool SCALED, bool RELU>
+__device__ inline void conv_epilogue_store(const floatx4 (&acc)[IB][JB], const ConvArgs &p,
+                                           long m0, int n0, int lane) {
+    const int r16 = lane & 15, g = lane >> 4;
+    OT *__restrict__ out = static_cast<OT *>(p.out);
+    constexpr bool Q8 = sizeof(OT) == 1;
+    auto finish = [&](float a, float cs, float bv) {
+        float v = SCALED ? a * cs + bv : a + bv;
+        if (RELU) v = v > 0.0f ? v : 0.0f;
+        if (Q8) v *= p.oscale;
+        return v;
+    };
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int col = n0 + 16 * j + r16;
+        const float bv = p.bias[col];
+        const float cs = SCALED ? p.colscale[col] : 1.0f;
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            if (POOL) {
+                float mx = fmaxf(fmaxf(acc[i][j][0], acc[i][j][1]), fmaxf(acc[i][j][2], acc[i][j][3]));
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                const long row = m0 + 16 * i + 4 * g;
+                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, finish(mx, cs, bv));
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) store<OT>(out + row * p.N + col, finish(acc[i][j][r], cs, bv));
+                }
+            }
+        }
+    }
+}
+
+template <typename OT, int IB, int JB, bool POOL, bool SPLIT, bool SCALED>
 __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArgs &p, long m0,
                                      int n0, int lane, unsigned split) {
-    const int r16 = lane & 15, g = lane >> 4;
     if (SPLIT) {
+        const int r16 = lane & 15, g = lane >> 4;
         float *__restrict__ part = p.partial + (long)split * p.M * p.N;
 #pragma unroll
         for (int i = 0; i < IB; ++i)
@@ -456,36 +502,8 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
             }
         return;
     }
-    OT *__restrict__ out = static_cast<OT *>(p.out);
-    constexpr bool Q8 = sizeof(OT) == 1;
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-        const int col = n0 + 16 * j + r16;
-        const float bv = p.bias[col];
-        const float cs = p.colscale ? p.colscale[col] : 1.0f;
-#pragma unroll
-        for (int i = 0; i < IB; ++i) {
-            float v[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                v[r] = p.colscale ? acc[i][j][r] * cs + bv : acc[i][j][r] + bv;
-                if (p.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
-                if (Q8) v[r] *= p.oscale;
-            }
-            if (POOL) {
-                float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
-                mx = fmaxf(mx, __shfl_xor(mx, 16));
-                const long row = m0 + 16 * i + 4 * g;
-                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, mx);
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const long row = m0 + 16 * i + 4 * g + r;
-                    if (row < p.M) store<OT>(out + row * p.N + col, v[r]);
-                }
-            }
-        }
-    }
+    if (p.relu) conv_epilogue_store<OT, IB, JB, POOL, SCALED, true>(acc, p, m0, n0, lane);
+    else conv_epilogue_store<OT, IB, JB, POOL, SCALED, false>(acc, p, m0, n0, lane);
 }
 
 // LDS-staged implicit GEMM.  A workgroup computes a BM x BN tile with four
@@ -556,6 +574,66 @@ struct KWalk {
                 ++unit;
             }
         }
+    }
+};
+
+// Byte offsets of a stage's input rows and weight rows from the tile's bases,
+// advanced stage by stage in the order of KWalk with scalar adds only: the
+// input offset of (unit, tap (kz, ky, kx), half) is kz*sz + ky*sy + kx*sx +
+// unit*ub + half*stage, the weight offset tap*tb + unit*ub + half*stage.
+// Offsets wrap modulo 2^32 in between; every stage's own offset is below 4 GB.
+struct CopyWalk {
+    int half = 0, kx = 0, ky = 0, kz = 0, half_n = 1, ks = 3;
+    unsigned a = 0, b = 0, sx = 0, sy = 0, sz = 0, tb = 0, ub = 128, stage = 128, wrap_b = 0;
+    __device__ void init(const KWalk &kw, int C, int D, int esz, int ksize) {
+        half_n = kw.half_n;
+        ks = ksize;
+        stage = (unsigned)kw.stage;
+        ub = (unsigned)kw.ub;
+        sx = (unsigned)(C * esz);
+        sy = sx * (unsigned)D;
+        sz = sy * (unsigned)D;
+        tb = sx;
+        wrap_b = ub - (unsigned)kw.taps * tb;
+        half = kw.half;
+        if (ks == 3) {
+            kz = kw.tap / 9;
+            ky = (kw.tap / 3) % 3;
+            kx = kw.tap % 3;
+        }
+        a = kz * sz + ky * sy + kx * sx + (unsigned)kw.unit * ub + (unsigned)half * stage;
+        b = (unsigned)kw.tap * tb + (unsigned)kw.unit * ub + (unsigned)half * stage;
+    }
+    __device__ void next() {
+        if (++half < half_n) {
+            a += stage;
+            b += stage;
+            return;
+        }
+        half = 0;
+        a -= (unsigned)(half_n - 1) * stage;
+        b -= (unsigned)(half_n - 1) * stage;
+        b += tb;
+        if (++kx < ks) {
+            a += sx;
+            return;
+        }
+        kx = 0;
+        a -= (unsigned)(ks - 1) * sx;
+        if (++ky < ks) {
+            a += sy;
+            return;
+        }
+        ky = 0;
+        a -= (unsigned)(ks - 1) * sy;
+        if (++kz < ks) {
+            a += sz;
+            return;
+        }
+        kz = 0;
+        a -= (unsigned)(ks - 1) * sz;
+        a += ub;
+        b += wrap_b;
     }
 };
 
@@ -697,7 +775,19 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
             if (s + 1 < s1) step(s + 1, ra0, rb0, ra1, rb1);
         }
     }
-    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+    conv_epilogue<OT, IB, JB, POOL, SPLIT, sizeof(T) == 1>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+}
+
+// One wave copies 64 16-byte pieces into consecutive LDS slots at `lds`: lane
+// l reads base + soff + voff(l) with a raw buffer load (the base in the
+// descriptor, the stage's offset in a scalar, the lane's in a 32-bit vector
+// register).  Descriptor word 3 0x00020000 is the gfx9 raw-buffer setting (32-bit
+// data format); 2^32 - 1 records leave every offset below 4 GB unclamped.
+__device__ inline void copy16_to_lds(const unsigned char *base, unsigned soff, unsigned voff,
+                                     unsigned char *lds) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, -1, 0x00020000),
+        (__attribute__((address_space(3))) void *)lds, 16, voff, soff, 0, 0);
 }
 
 // s_waitcnt with vmcnt(n) and no wait on the other counters (gfx9 encoding:
@@ -732,7 +822,10 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
     constexpr int IB = WTM / 16, JB = WTN / 16;
     constexpr int SLOTS = (BM + BN) * PCS;  // 16-byte slots per stage
     constexpr int J = SLOTS / 256;          // load instructions per wave per stage
-    static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0 && (BM * PCS) % 64 == 0,
+    // JA loads of each wave fill A rows, the rest B rows (the A slots are a
+    // whole number of four-wave load rounds)
+    constexpr int JA = BM * PCS / 256;
+    static_assert(SLOTS % 256 == 0 && (64 / PCS) <= 16 && BM % 16 == 0 && (BM * PCS) % 256 == 0,
                   "stage layout");
     static_assert(NST == 2 || NST == 3, "ring depth");
     static_assert((NST - 2) * J < 64, "vmcnt range");
@@ -764,17 +857,14 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
         static_cast<const unsigned char *>(p.w) + (long)nb * K * (long)sizeof(T);
 
     // load j of this wave fills slots (4j + wave)*64 + lane of a stage
-    // buffer; the 64 slots of one load lie in A rows or all in B rows (BM *
-    // PCS is a multiple of 64), so the role is uniform
+    // buffer: A rows for j < JA, B rows after
     unsigned voff[J];
-    bool isa[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         const int slot = (4 * j + wave) * 64 + lane;
         const int row = slot / PCS, pl = slot % PCS;
         const int ps = pl ^ swz<STAGE>(row);
-        isa[j] = (4 * j + wave) * 64 < BM * PCS;
-        if (isa[j]) {
+        if (j < JA) {
             const long m = min(mb + row, p.M - 1);
             voff[j] = (unsigned)((row_voxel<POOL>(m, p) - vb0) * C * (long)sizeof(T)) + ps * 16;
         } else {
@@ -798,28 +888,19 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
             b_off[j][h] = row * STAGE + (((g + 4 * h) ^ swz<STAGE>(row)) << 4);
     }
 
-    KWalk kw;  // the next stage to copy
+    KWalk kw;
     const int ns = kw.init(C, (int)sizeof(T), STAGE, p.taps);
     int s0 = 0, s1 = ns;
     if (SPLIT) kw.split(ns, tz, p.splits, &s0, &s1);
     kw.seek(s0);
+    CopyWalk cw;  // the next stage to copy
+    cw.init(kw, C, D, (int)sizeof(T), ks);
     auto issue = [&](unsigned char *buf) {
-        const int ftap = kw.tap, fc0 = kw.c0();
-        int kz = 0, ky = 0, kx = 0;
-        if (ks == 3) {
-            kz = ftap / 9;
-            ky = (ftap / 3) % 3;
-            kx = ftap % 3;
-        }
-        const long aoff = ((long)((kz * D + ky) * D + kx) * C + fc0) * (long)sizeof(T);
-        const long boff = ((long)ftap * C + fc0) * (long)sizeof(T);
-        const unsigned char *const sa = a_base + aoff, *const sb = b_base + boff;
 #pragma unroll
         for (int j = 0; j < J; ++j)
-            __builtin_amdgcn_global_load_lds(
-                (const void *)((isa[j] ? sa : sb) + voff[j]),
-                (__attribute__((address_space(3))) void *)(buf + (4 * j + wave) * 1024), 16, 0, 0);
-        kw.next();
+            copy16_to_lds(j < JA ? a_base : b_base, j < JA ? cw.a : cw.b, voff[j],
+                          buf + (4 * j + wave) * 1024);
+        cw.next();
     };
 
     floatx4 acc[IB][JB];
@@ -861,7 +942,202 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
             }
         }
     }
-    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+    conv_epilogue<OT, IB, JB, POOL, SPLIT, sizeof(T) == 1>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+}
+
+// Pooled 3x3x3 convolution (16-bit, 128-byte channel units) staging the
+// input rows of a (kz, ky) tap group once for its three kx taps.
+//
+// A tile is kPoolWin consecutive pool windows (8 rows each, window element
+// (dz, dy, dx) innermost, as row_voxel orders them) by 64 columns; four waves
+// along m own 48 rows each.  The windows of one window line (fixed batch
+// item, pz, py) take consecutive "slots", and each line one spare slot
+// after its last window, so slot s = window + window / PD.  For a (kz, ky)
+// group the tile stages, per (dz, dy) plane, two rows per slot: the input
+// voxels at x = 2 px + xo (xo = 0, 1) of the slot (the spare slot reads x =
+// OD, OD + 1, inside the input).  Row (window, dz, dy, dx) at tap kx then
+// reads staged row plane*64 + 2 (slot - slot0) + dx + kx -- the kx = 0 row
+// shifted by kx -- so the group moves 4 x 56 rows of input into LDS instead
+// of 3 x 192.  Weights keep their own two-buffer ring, one tap per step.
+// Each output still sums (unit, tap, half) in KWalk's order with the same
+// MFMA sequence, so the results are bit-identical to the other variants.
+//
+// Staged rows are XOR-swizzled by (plane, row parity): the 16 rows a
+// fragment read touches are pairs of adjacent rows in the four planes, and
+// (plane, parity) keeps every 8-lane group on distinct 16-byte bank slots for
+// every shift kx.
+constexpr int kPoolWin = 24;           // windows per tile
+constexpr int kPoolPlane = 64;         // LDS rows per (dz, dy) plane
+constexpr int kPoolStaged = 56;        // rows loaded per plane (two per slot)
+__device__ inline int swz_plane(int r) { return (((r >> 6) << 1) | (r & 1)) & 7; }
+
+template <typename T, typename OT>
+__global__ __launch_bounds__(256, 2) void conv_pool3_kernel(ConvArgs p) {
+    static_assert(sizeof(T) == 2, "16-bit operands");
+    constexpr int BM = 8 * kPoolWin, BN = 64, WTM = BM / 4, IB = WTM / 16, JB = BN / 16;
+    constexpr int ASZ = 4 * kPoolPlane * 128, BSZ = BN * 128;
+    constexpr int JA = 4 * kPoolStaged * 8 / 256;  // A loads per wave per group (7)
+    constexpr int JW = BN * 8 / 256;               // weight loads per wave per step (2)
+    constexpr int PARTS = kPoolStaged * 8 / 64;    // 64-slot loads per plane (7)
+    static_assert(WTM % 16 == 0 && 4 * PARTS == 4 * JA && JA == PARTS, "pooled tile layout");
+    __shared__ __attribute__((aligned(16))) unsigned char A0[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char A1[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B0[BSZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B1[BSZ];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    unsigned tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const long mb = (long)tx * BM;
+    const int nb = ty * BN;
+    const int C = p.C, D = p.D, PD = p.OD >> 1;
+    const unsigned csz = (unsigned)C * (unsigned)sizeof(T);
+    const long K = (long)p.taps * C;
+
+    // the tile's windows wi0 .. wl and the slots they read, s0 .. shi
+    const unsigned nwin = (unsigned)(p.M >> 3), wi0 = (unsigned)(mb >> 3);
+    const unsigned wl = min(wi0 + (unsigned)kPoolWin, nwin) - 1;
+    const unsigned s0 = wi0 + row_div(wi0, p), shi = wl + row_div(wl, p) + 1;
+    // The slots s0 .. shi lie on at most three window lines (24 windows, at
+    // least 12 per line): decode the lines' first voxels once (uniform).
+    const unsigned l0 = slot_div(s0, p);
+    long lbase[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const unsigned line = l0 + k, q1 = row_div(line, p), q2 = row_div(q1, p);
+        const int py = (int)(line - q1 * (unsigned)PD), pz = (int)(q1 - q2 * (unsigned)PD);
+        lbase[k] = (((long)q2 * D + 2 * pz) * D + 2 * py) * (long)D;
+    }
+    // input voxel of slot s, plane (dz, dy) and x offset xo
+    auto slot_voxel = [&](unsigned s, int plane, int xo) -> long {
+        const unsigned line = slot_div(s, p), k = line - l0;
+        const int px = (int)(s - line * (unsigned)(PD + 1));
+        const long lb = k == 0 ? lbase[0] : k == 1 ? lbase[1] : lbase[2];
+        return lb + ((plane >> 1) * D + (plane & 1)) * (long)D + 2 * px + xo;
+    };
+    const long vb0 = slot_voxel(s0, 0, 0);  // the smallest voxel the tile stages
+    const unsigned char *__restrict__ a_base =
+        static_cast<const unsigned char *>(p.in) + vb0 * (long)csz;
+    const unsigned char *__restrict__ b_base =
+        static_cast<const unsigned char *>(p.w) + (long)nb * K * (long)sizeof(T);
+
+    // A load j of this wave fills part (L % PARTS) of plane L / PARTS, L = 4j + wave
+    unsigned avoff[JA];
+#pragma unroll
+    for (int j = 0; j < JA; ++j) {
+        const int L = 4 * j + wave, plane = L / PARTS, part = L % PARTS;
+        const int q = (part * 64 + lane) >> 3, pl = lane & 7, r = plane * kPoolPlane + q;
+        const unsigned s = min(s0 + (unsigned)(q >> 1), shi);
+        avoff[j] = (unsigned)((slot_voxel(s, plane, q & 1) - vb0) * (long)csz) +
+                   ((pl ^ swz_plane(r)) << 4);
+    }
+    unsigned bvoff[JW];
+#pragma unroll
+    for (int j = 0; j < JW; ++j) {
+        const int slot = (4 * j + wave) * 64 + lane, row = slot >> 3, pl = slot & 7;
+        bvoff[j] = (unsigned)((long)row * K * (long)sizeof(T)) + ((pl ^ swz<128>(row)) << 4);
+    }
+    const int r16 = lane & 15, g = lane >> 4;
+    int aoff[IB][3][2], boff[JB][2];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+        const long m = min(mb + wave * WTM + 16 * i + r16, p.M - 1);
+        const unsigned wi = (unsigned)(m >> 3);
+        const int w = (int)(m & 7);
+        const int r0 = (w >> 1) * kPoolPlane + 2 * (int)(wi + row_div(wi, p) - s0) + (w & 1);
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                aoff[i][kx][h] = (r0 + kx) * 128 + (((g + 4 * h) ^ swz_plane(r0 + kx)) << 4);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int row = 16 * j + r16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) boff[j][h] = row * 128 + (((g + 4 * h) ^ swz<128>(row)) << 4);
+    }
+
+    // copy walks: the A group (unit, kz, ky) and the weight step (unit, tap)
+    const int units = (int)(csz / 128u), ng = 9 * units, nsteps = 27 * units;
+    const unsigned dy_b = (unsigned)D * csz, dz_b = dy_b * (unsigned)D;
+    unsigned a_off = 0, b_off = 0;
+    int a_kz = 0, a_ky = 0, b_tap = 0;
+    auto issue_a = [&](unsigned char *buf) {
+#pragma unroll
+        for (int j = 0; j < JA; ++j) {
+            const int L = 4 * j + wave;
+            copy16_to_lds(a_base, a_off, avoff[j], buf + (L / PARTS) * (kPoolPlane * 128) +
+                                                       (L % PARTS) * 1024);
+        }
+        if (++a_ky < 3) {
+            a_off += dy_b;
+        } else {
+            a_ky = 0;
+            a_off -= 2 * dy_b;
+            if (++a_kz < 3) {
+                a_off += dz_b;
+            } else {
+                a_kz = 0;
+                a_off = a_off - 2 * dz_b + 128u;
+            }
+        }
+    };
+    auto issue_b = [&](unsigned char *buf) {
+#pragma unroll
+        for (int j = 0; j < JW; ++j)
+            copy16_to_lds(b_base, b_off, bvoff[j], buf + (4 * j + wave) * 1024);
+        if (++b_tap < 27) {
+            b_off += csz;
+        } else {
+            b_tap = 0;
+            b_off = b_off - 26 * csz + 128u;
+        }
+    };
+
+    floatx4 acc[IB][JB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto compute = [&](const unsigned char *abuf, const unsigned char *bbuf, const int kx) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint4 a[IB], b[JB];
+#pragma unroll
+            for (int i = 0; i < IB; ++i) a[i] = *reinterpret_cast<const uint4 *>(abuf + aoff[i][kx][h]);
+#pragma unroll
+            for (int j = 0; j < JB; ++j) b[j] = *reinterpret_cast<const uint4 *>(bbuf + boff[j][h]);
+            Mfma<T>::chunk(acc, a, b);
+        }
+    };
+    // Step t = 3 gi + kx of group gi: the weights of step t were copied at
+    // step t - 1 (before that step's group copy), the group's rows at the
+    // group's first step - 3.  At kx = 1 the next group's copies, issued after
+    // this step's weights, may still be in flight.
+    auto group = [&](const int gi, unsigned char *acur, unsigned char *anxt, unsigned char *bev,
+                     unsigned char *bod) {
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+            const int t = 3 * gi + kx;
+            // weight buffer of step t: even steps bev, odd bod (gi parity folded in by the caller)
+            unsigned char *bcur = (kx & 1) ? bod : bev, *bnxt = (kx & 1) ? bev : bod;
+            if (kx == 1 && gi + 1 < ng) wait_vmcnt<JA>();
+            else wait_vmcnt<0>();
+            __builtin_amdgcn_s_barrier();
+            if (t + 1 < nsteps) issue_b(bnxt);
+            if (kx == 0 && gi + 1 < ng) issue_a(anxt);
+            compute(acur, bcur, kx);
+        }
+    };
+    issue_b(B0);
+    issue_a(A0);
+    for (int gi = 0; gi < ng; gi += 2) {
+        group(gi, A0, A1, B0, B1);                    // steps 3gi.. start even
+        if (gi + 1 < ng) group(gi + 1, A1, A0, B1, B0);  // steps 3gi+3.. start odd
+    }
+    conv_epilogue<OT, IB, JB, true, false, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
 }
 
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
@@ -1182,6 +1458,7 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
     p.M = (long)batch * d.OD * d.OD * d.OD;
     p.small_rows = p.M < (1L << 31) ? 1 : 0;
     fast_div(d.pool ? d.OD / 2 : d.OD, &p.rdiv_mul, &p.rdiv_sh);
+    fast_div((d.pool ? d.OD / 2 : d.OD) + 1, &p.sdiv_mul, &p.sdiv_sh);
     const bool lds = tdm::conv_uses_lds(dtype_of<T>(), d);
     if (lds) {
         int v = variant;
@@ -1215,6 +1492,18 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
         case 11: TDM_CONV_DMA(2, 128, 32, 64, 3); break;
         case 12: TDM_CONV_DMA(2, 64, 64, 128, 2); break;
         case 13: TDM_CONV_DMA(4, 64, 64, 128, 2); break;
+        case 14:
+            // the pooled tap-group kernel runs unsplit pooled layers with
+            // 31-bit rows; the same layer split (or wider) runs variant 13
+            if constexpr (sizeof(T) == 2) {
+                if (splits == 1 && d.pool && p.small_rows) {
+                    dim3 grid((unsigned)((p.M + 8 * kPoolWin - 1) / (8 * kPoolWin)), d.N / 64, 1);
+                    conv_pool3_kernel<T, OT><<<grid, 256, 0, s>>>(p);
+                    break;
+                }
+            }
+            TDM_CONV_DMA(4, 64, 64, 128, 2);
+            break;
         case 0: TDM_CONV_LDS(2, 64, 32, 128); break;
         case 1: TDM_CONV_LDS(2, 64, 64, 128); break;
         case 2: TDM_CONV_LDS(4, 64, 64, 128); break;
@@ -1291,12 +1580,13 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
 
 // Tile variants: {waves along m, wave tile m, wave tile n, stage bytes};
 // 0-8 stage through VGPRs (conv_lds_kernel), 9-13 copy straight into LDS
-// (conv_dma_kernel; 9-11 with three stage buffers, 12-13 with two).
+// (conv_dma_kernel; 9-11 with three stage buffers, 12-13 with two), 14 the
+// pooled tap-group kernel (conv_pool3_kernel).
 namespace {
 constexpr int kVariants[kConvVariants][4] = {
     {2, 64, 32, 128},  {2, 64, 64, 128}, {4, 64, 64, 128}, {2, 128, 32, 128}, {2, 128, 64, 128},
     {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64},  {4, 64, 64, 64},
-    {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}};
+    {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}, {4, 48, 64, 128}};
 }
 
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
@@ -1304,6 +1594,11 @@ bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     const int bn = (4 / kVariants[v][0]) * kVariants[v][2];
     const int stage_k = kVariants[v][3] / (int)dtype_size(dt);
     if (d.N % bn != 0 || d.C % stage_k != 0) return false;
+    // pooled tap groups: 16-bit, a fused pool after a 3x3x3 window, and
+    // pooled extents of at least 12 (a tile's 24 windows then read at most
+    // 27 slots of their window lines)
+    if (v == 14 && (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 || d.OD / 2 < 12))
+        return false;
     if (v >= 9) {
         // conv_dma_kernel addresses a tile's input rows and weight rows with
         // 32-bit byte offsets from scalar bases: the input of the cubes a
