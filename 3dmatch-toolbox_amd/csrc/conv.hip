@@ -442,52 +442,12 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs p) {
 
 // Epilogue of a wave tile of IB x JB 16x16 blocks at (m0, n0): the lane holds
 // rows m0+16i+4g+r (r = 0..3) of column n0+16j+r16.  Split-K writes float
-// partials; otherwise bias (after the fp8 column scale when SCALED), ReLU,
-// the fused 2x2x2 pool and the store.  The pool takes the window's largest
-// accumulator first and applies scale, bias and ReLU to it once: each of
-// those is monotone non-decreasing (rounding included, the scales are
-// non-negative), so this equals the largest of the transformed values bit
-// for bit.
-template <typename OT, int IB, int JB, bool POOL, bool SCALED, bool RELU>
-__device__ inline void conv_epilogue_store(const floatx4 (&acc)[IB][JB], const ConvArgs &p,
-                                           long m0, int n0, int lane) {
-    const int r16 = lane & 15, g = lane >> 4;
-    OT *__restrict__ out = static_cast<OT *>(p.out);
-    constexpr bool Q8 = sizeof(OT) == 1;
-    auto finish = [&](float a, float cs, float bv) {
-        float v = SCALED ? a * cs + bv : a + bv;
-        if (RELU) v = v > 0.0f ? v : 0.0f;
-        if (Q8) v *= p.oscale;
-        return v;
-    };
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-        const int col = n0 + 16 * j + r16;
-        const float bv = p.bias[col];
-        const float cs = SCALED ? p.colscale[col] : 1.0f;
-#pragma unroll
-        for (int i = 0; i < IB; ++i) {
-            if (POOL) {
-                float mx = fmaxf(fmaxf(acc[i][j][0], acc[i][j][1]), fmaxf(acc[i][j][2], acc[i][j][3]));
-                mx = fmaxf(mx, __shfl_xor(mx, 16));
-                const long row = m0 + 16 * i + 4 * g;
-                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, finish(mx, cs, bv));
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const long row = m0 + 16 * i + 4 * g + r;
-                    if (row < p.M) store<OT>(out + row * p.N + col, finish(acc[i][j][r], cs, bv));
-                }
-            }
-        }
-    }
-}
-
-template <typename OT, int IB, int JB, bool POOL, bool SPLIT, bool SCALED>
+// partials; otherwise bias, ReLU, the fused 2x2x2 pool and the store.
+template <typename OT, int IB, int JB, bool POOL, bool SPLIT>
 __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArgs &p, long m0,
                                      int n0, int lane, unsigned split) {
+    const int r16 = lane & 15, g = lane >> 4;
     if (SPLIT) {
-        const int r16 = lane & 15, g = lane >> 4;
         float *__restrict__ part = p.partial + (long)split * p.M * p.N;
 #pragma unroll
         for (int i = 0; i < IB; ++i)
@@ -502,8 +462,36 @@ __device__ inline void conv_epilogue(const floatx4 (&acc)[IB][JB], const ConvArg
             }
         return;
     }
-    if (p.relu) conv_epilogue_store<OT, IB, JB, POOL, SCALED, true>(acc, p, m0, n0, lane);
-    else conv_epilogue_store<OT, IB, JB, POOL, SCALED, false>(acc, p, m0, n0, lane);
+    OT *__restrict__ out = static_cast<OT *>(p.out);
+    constexpr bool Q8 = sizeof(OT) == 1;
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int col = n0 + 16 * j + r16;
+        const float bv = p.bias[col];
+        const float cs = p.colscale ? p.colscale[col] : 1.0f;
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = p.colscale ? acc[i][j][r] * cs + bv : acc[i][j][r] + bv;
+                if (p.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+                if (Q8) v[r] *= p.oscale;
+            }
+            if (POOL) {
+                float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                const long row = m0 + 16 * i + 4 * g;
+                if ((g & 1) == 0 && row < p.M) store<OT>(out + (row >> 3) * p.N + col, mx);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long row = m0 + 16 * i + 4 * g + r;
+                    if (row < p.M) store<OT>(out + row * p.N + col, v[r]);
+                }
+            }
+        }
+    }
 }
 
 // LDS-staged implicit GEMM.  A workgroup computes a BM x BN tile with four
@@ -775,7 +763,7 @@ __global__ __launch_bounds__(256, (Tile<WMW, WTM, WTN, STAGE>::OCC)) void conv_l
             if (s + 1 < s1) step(s + 1, ra0, rb0, ra1, rb1);
         }
     }
-    conv_epilogue<OT, IB, JB, POOL, SPLIT, sizeof(T) == 1>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
 }
 
 // One wave copies 64 16-byte pieces into consecutive LDS slots at `lds`: lane
@@ -942,7 +930,7 @@ __global__ __launch_bounds__(256, 2) void conv_dma_kernel(ConvArgs p) {
             }
         }
     }
-    conv_epilogue<OT, IB, JB, POOL, SPLIT, sizeof(T) == 1>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
+    conv_epilogue<OT, IB, JB, POOL, SPLIT>(acc, p, mb + wm * WTM, nb + wn * WTN, lane, tz);
 }
 
 // Pooled 3x3x3 convolution (16-bit, 128-byte channel units) staging the
@@ -1137,7 +1125,7 @@ __global__ __launch_bounds__(256, 2) void conv_pool3_kernel(ConvArgs p) {
         group(gi, A0, A1, B0, B1);                    // steps 3gi.. start even
         if (gi + 1 < ng) group(gi + 1, A1, A0, B1, B0);  // steps 3gi+3.. start odd
     }
-    conv_epilogue<OT, IB, JB, true, false, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
+    conv_epilogue<OT, IB, JB, true, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
 }
 
 // Sum of split-K partials + bias (+ReLU), converted to the output type.
@@ -1270,6 +1258,15 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
         if (x < 0 || y < 0 || z < 0 || x >= ex[i] || y >= ey[i] || z >= ez[i]) return 0.0f;
         return vb[i][kz * sz[i] + ky * sy[i] + kx];
     };
+    // Patch and single-grid modes share one pair of pitches, so a lane's tap
+    // offsets are computed once (32-bit: the host checks that a tap offset
+    // fits), and a row block whose rows all have their 3x3x3 neighbourhood
+    // inside the grid (a wave-wide vote, so the branch is uniform) reads its
+    // taps without bounds tests.
+    constexpr bool UNI = MODE != 2;
+    const int usy = MODE == 0 ? a.P : a.dx, usz = MODE == 0 ? a.P * a.P : a.dx * a.dy;
+    auto tap_off = [&](int t) { return (t / 9) * usz + ((t / 3) % 3) * usy + t % 3; };
+    auto block_in = [&](int i) { return !GRID || __all(inb[i]); };
     floatx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1281,13 +1278,24 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
         // 7 k-steps of 4 taps: lane group g supplies tap 4s + g.  The tap
         // number depends on the lane, so its (kz, ky, kx) is decoded once
         // here, not once per row block.
+        bool fast[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fast[i] = UNI && block_in(i);
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             const int t = 4 * s + g;
             const int kz = t / 9, ky = (t / 3) % 3, kx = t % 3;
+            const int to = t < 27 ? tap_off(t) : 0;
             float av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = t < 27 ? tap_value(i, kz, ky, kx) : 0.0f;
+            for (int i = 0; i < 4; ++i) {
+                if (fast[i]) {
+                    const float v = vb[i][to];
+                    av[i] = t < 27 ? v : 0.0f;
+                } else {
+                    av[i] = t < 27 ? tap_value(i, kz, ky, kx) : 0.0f;
+                }
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float *wf = reinterpret_cast<const float *>(w);
@@ -1311,14 +1319,25 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
             tk[e][1] = (t / 3) % 3;
             tk[e][2] = t % 3;
         }
+        int to[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) to[e] = 8 * g + e < 27 ? tap_off(8 * g + e) : 0;
         uint4 af[4], bf[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             unsigned short hv[8];
+            if (UNI && block_in(i)) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int t = 8 * g + e;
-                hv[e] = bits16<T>(t < 27 ? tap_value(i, tk[e][0], tk[e][1], tk[e][2]) : 0.0f);
+                for (int e = 0; e < 8; ++e) {
+                    const float v = vb[i][to[e]];
+                    hv[e] = bits16<T>(8 * g + e < 27 ? v : 0.0f);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int t = 8 * g + e;
+                    hv[e] = bits16<T>(t < 27 ? tap_value(i, tk[e][0], tk[e][1], tk[e][2]) : 0.0f);
+                }
             }
             __builtin_memcpy(&af[i], hv, 16);
         }
@@ -1692,6 +1711,9 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
     a.grids = src.grids;
     a.gid = src.gid;
     const int mode = src.kp == nullptr ? 0 : src.grids == nullptr ? 1 : 2;
+    // the kernel forms single-grid tap offsets in 32 bits
+    TDM_CHECK(mode != 1 || 2L * a.dx * a.dy + 2L * a.dx + 2 < (1L << 31), TDM_ERR_INVALID_ARG,
+              "grid slices of %d x %d voxels are too large", a.dx, a.dy);
 #define TDM_C1(T, OT)                                                                         \
     do {                                                                                      \
         if (mode == 2) conv1_kernel<T, OT, 2><<<grid, 256, 0, s>>>(a);                        \

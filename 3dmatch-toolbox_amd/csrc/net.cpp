@@ -722,28 +722,38 @@ int tdm_net_autotune(tdm_net *net, int num, void *stream) {
         const size_t nq = (size_t)tdm::conv_reduction_steps(net->dt, d);
         const int splits = choose_splits(*net, l, nq);
         const bool out_f32 = net->dt == tdm::DType::F32 || li + 1 == net->layers.size();
-        float best = 3.4e38f;
-        int best_v = tdm::conv_default_variant(net->dt, d);
-        for (int v = 0; v < tdm::kConvVariants && st == TDM_OK; ++v) {
-            if (!tdm::conv_variant_valid(net->dt, v, d)) continue;
-            auto run = [&]() {
-                return tdm::launch_conv(net->dt, out_f32, d, n, net->act[0], l.w_dev, l.b_dev,
-                                        net->act[1], splits, net->partial, v, s, l.colscale_dev,
-                                        out_scale(*net, l));
-            };
-            if ((st = run()) != TDM_OK) break;  // warm-up
-            if (hipEventRecord(e0, s) != hipSuccess) st = TDM_ERR_HIP;
-            for (int r = 0; r < 3 && st == TDM_OK; ++r) st = run();
-            if (st == TDM_OK && hipEventRecord(e1, s) != hipSuccess) st = TDM_ERR_HIP;
-            float ms = 0.0f;
-            if (st == TDM_OK && (hipEventSynchronize(e1) != hipSuccess ||
-                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                st = TDM_ERR_HIP;
-            if (st == TDM_OK && ms < best) {
-                best = ms;
-                best_v = v;
+        auto run = [&](int v) {
+            return tdm::launch_conv(net->dt, out_f32, d, n, net->act[0], l.w_dev, l.b_dev,
+                                    net->act[1], splits, net->partial, v, s, l.colscale_dev,
+                                    out_scale(*net, l));
+        };
+        // Three rounds over the candidates, each timing one launch per
+        // candidate, and every candidate's fastest launch decides: drifting
+        // clocks then weigh on all candidates alike, not on the last ones.
+        std::vector<int> cand;
+        for (int v = 0; v < tdm::kConvVariants; ++v)
+            if (tdm::conv_variant_valid(net->dt, v, d)) cand.push_back(v);
+        std::vector<float> fastest(cand.size(), 3.4e38f);
+        for (size_t c = 0; c < cand.size() && st == TDM_OK; ++c) st = run(cand[c]);  // warm-up
+        for (int r = 0; r < 3 && st == TDM_OK; ++r) {
+            for (size_t c = 0; c < cand.size() && st == TDM_OK; ++c) {
+                if (hipEventRecord(e0, s) != hipSuccess) st = TDM_ERR_HIP;
+                if (st == TDM_OK) st = run(cand[c]);
+                if (st == TDM_OK && hipEventRecord(e1, s) != hipSuccess) st = TDM_ERR_HIP;
+                float ms = 0.0f;
+                if (st == TDM_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    st = TDM_ERR_HIP;
+                if (st == TDM_OK) fastest[c] = std::min(fastest[c], ms);
             }
         }
+        int best_v = tdm::conv_default_variant(net->dt, d);
+        float best = 3.4e38f;
+        for (size_t c = 0; c < cand.size(); ++c)
+            if (fastest[c] < best) {
+                best = fastest[c];
+                best_v = cand[c];
+            }
         l.variant = best_v;
     }
     (void)hipEventDestroy(e0);
