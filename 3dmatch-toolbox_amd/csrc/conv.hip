@@ -1258,15 +1258,22 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
         if (x < 0 || y < 0 || z < 0 || x >= ex[i] || y >= ey[i] || z >= ez[i]) return 0.0f;
         return vb[i][kz * sz[i] + ky * sy[i] + kx];
     };
-    // Patch and single-grid modes share one pair of pitches, so a lane's tap
-    // offsets are computed once (32-bit: the host checks that a tap offset
-    // fits), and a row block whose rows all have their 3x3x3 neighbourhood
-    // inside the grid (a wave-wide vote, so the branch is uniform) reads its
-    // taps without bounds tests.
+    // A row block whose rows all have their 3x3x3 neighbourhood inside the
+    // grid (a wave-wide vote, so the branch is uniform) reads its taps without
+    // bounds tests, at 32-bit tap offsets.  Patch and single-grid modes share
+    // one pair of pitches, so a lane's offsets are computed once (the host
+    // checks that they fit); with a table of grids each row's own grid gives
+    // them, and a block takes this path only if its grids' slices are below
+    // 2^30 voxels.
     constexpr bool UNI = MODE != 2;
     const int usy = MODE == 0 ? a.P : a.dx, usz = MODE == 0 ? a.P * a.P : a.dx * a.dy;
     auto tap_off = [&](int t) { return (t / 9) * usz + ((t / 3) % 3) * usy + t % 3; };
-    auto block_in = [&](int i) { return !GRID || __all(inb[i]); };
+    auto block_in = [&](int i) {
+        return !GRID || __all(inb[i] && (UNI || sz[i] < (1L << 30)));
+    };
+    auto row_tap_off = [&](int i, int kz, int ky, int kx) {
+        return kz * (int)sz[i] + ky * sy[i] + kx;
+    };
     floatx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -1280,7 +1287,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
         // here, not once per row block.
         bool fast[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) fast[i] = UNI && block_in(i);
+        for (int i = 0; i < 4; ++i) fast[i] = block_in(i);
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             const int t = 4 * s + g;
@@ -1290,7 +1297,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (fast[i]) {
-                    const float v = vb[i][to];
+                    const float v = vb[i][UNI ? to : t < 27 ? row_tap_off(i, kz, ky, kx) : 0];
                     av[i] = t < 27 ? v : 0.0f;
                 } else {
                     av[i] = t < 27 ? tap_value(i, kz, ky, kx) : 0.0f;
@@ -1326,10 +1333,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             unsigned short hv[8];
-            if (UNI && block_in(i)) {
+            if (block_in(i)) {
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const float v = vb[i][to[e]];
+                    const int te = 8 * g + e;
+                    const float v = vb[i][UNI ? to[e]
+                                          : te < 27 ? row_tap_off(i, tk[e][0], tk[e][1], tk[e][2])
+                                                    : 0];
                     hv[e] = bits16<T>(8 * g + e < 27 ? v : 0.0f);
                 }
             } else {
