@@ -989,20 +989,14 @@ __global__ __launch_bounds__(256, 2) void conv_pool3_kernel(ConvArgs p) {
     const unsigned s0 = wi0 + row_div(wi0, p), shi = wl + row_div(wl, p) + 1;
     // The slots s0 .. shi lie on at most three window lines (24 windows, at
     // least 12 per line): decode the lines' first voxels once (uniform).
-    const unsigned l0 = slot_div(s0, p);
-    long lbase[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const unsigned line = l0 + k, q1 = row_div(line, p), q2 = row_div(q1, p);
-        const int py = (int)(line - q1 * (unsigned)PD), pz = (int)(q1 - q2 * (unsigned)PD);
-        lbase[k] = (((long)q2 * D + 2 * pz) * D + 2 * py) * (long)D;
-    }
     // input voxel of slot s, plane (dz, dy) and x offset xo
     auto slot_voxel = [&](unsigned s, int plane, int xo) -> long {
-        const unsigned line = slot_div(s, p), k = line - l0;
+        const unsigned line = slot_div(s, p);
         const int px = (int)(s - line * (unsigned)(PD + 1));
-        const long lb = k == 0 ? lbase[0] : k == 1 ? lbase[1] : lbase[2];
-        return lb + ((plane >> 1) * D + (plane & 1)) * (long)D + 2 * px + xo;
+        const unsigned q1 = row_div(line, p), q2 = row_div(q1, p);
+        const int py = (int)(line - q1 * (unsigned)PD), pz = (int)(q1 - q2 * (unsigned)PD);
+        return (((long)q2 * D + 2 * pz + (plane >> 1)) * D + 2 * py + (plane & 1)) * (long)D +
+               2 * px + xo;
     };
     const long vb0 = slot_voxel(s0, 0, 0);  // the smallest voxel the tile stages
     const unsigned char *__restrict__ a_base =
@@ -1124,6 +1118,174 @@ __global__ __launch_bounds__(256, 2) void conv_pool3_kernel(ConvArgs p) {
     for (int gi = 0; gi < ng; gi += 2) {
         group(gi, A0, A1, B0, B1);                    // steps 3gi.. start even
         if (gi + 1 < ng) group(gi + 1, A1, A0, B1, B0);  // steps 3gi+3.. start odd
+    }
+    conv_epilogue<OT, IB, JB, true, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
+}
+
+// The pooled tap-group kernel with one workgroup of eight waves per CU and a
+// whole (kz, ky) group -- its input rows and the weights of its three kx taps
+// -- per stage buffer: 48 windows x 64 columns per tile, eight waves of 48
+// rows, so one barrier publishes three taps of work (against one per tap)
+// and the copies of the next group have a whole group of MFMAs to land.  The
+// two-buffer ring takes the CU's whole 160 KB of LDS (2 x 56 KB of input rows,
+// 4 planes x 112 rows, and 2 x 24 KB of weights).  Same row order, staging,
+// swizzle and reduction order as conv_pool3_kernel (bit-identical).
+constexpr int kPoolWinG = 48;     // windows per tile
+constexpr int kPoolPlaneG = 112;  // LDS rows per (dz, dy) plane, all loaded
+
+template <typename T, typename OT>
+__global__ __launch_bounds__(512, 1) void conv_pool3g_kernel(ConvArgs p) {
+    static_assert(sizeof(T) == 2, "16-bit operands");
+    constexpr int WAVES = 8, BM = 8 * kPoolWinG, BN = 64, WTM = BM / WAVES, IB = WTM / 16,
+                  JB = BN / 16;
+    constexpr int ASZ = 4 * kPoolPlaneG * 128, BSZ = 3 * BN * 128;
+    constexpr int PARTS = kPoolPlaneG * 8 / 64;   // 64-slot loads per plane (14)
+    constexpr int JA = 4 * PARTS / WAVES;         // input-row loads per wave per group (7)
+    constexpr int BPARTS = BN * 8 / 64;           // 64-slot loads per tap of weights (8)
+    constexpr int JW = 3 * BPARTS / WAVES;        // weight loads per wave per group (3)
+    static_assert(WTM % 16 == 0 && (4 * PARTS) % WAVES == 0 && (3 * BPARTS) % WAVES == 0,
+                  "pooled group tile layout");
+    __shared__ __attribute__((aligned(16))) unsigned char A0[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char A1[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B0[BSZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B1[BSZ];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    unsigned tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const long mb = (long)tx * BM;
+    const int nb = ty * BN;
+    const int C = p.C, D = p.D, PD = p.OD >> 1;
+    const unsigned csz = (unsigned)C * (unsigned)sizeof(T);
+    const long K = (long)p.taps * C;
+
+    const unsigned nwin = (unsigned)(p.M >> 3), wi0 = (unsigned)(mb >> 3);
+    const unsigned wl = min(wi0 + (unsigned)kPoolWinG, nwin) - 1;
+    const unsigned s0 = wi0 + row_div(wi0, p), shi = wl + row_div(wl, p) + 1;
+    auto slot_voxel = [&](unsigned s, int plane, int xo) -> long {
+        const unsigned line = slot_div(s, p);
+        const int px = (int)(s - line * (unsigned)(PD + 1));
+        const unsigned q1 = row_div(line, p), q2 = row_div(q1, p);
+        const int py = (int)(line - q1 * (unsigned)PD), pz = (int)(q1 - q2 * (unsigned)PD);
+        return (((long)q2 * D + 2 * pz + (plane >> 1)) * D + 2 * py + (plane & 1)) * (long)D +
+               2 * px + xo;
+    };
+    const long vb0 = slot_voxel(s0, 0, 0);
+    const unsigned char *__restrict__ a_base =
+        static_cast<const unsigned char *>(p.in) + vb0 * (long)csz;
+    const unsigned char *__restrict__ b_base =
+        static_cast<const unsigned char *>(p.w) + (long)nb * K * (long)sizeof(T);
+    auto swz_g = [](int plane, int r) { return ((plane << 1) | (r & 1)) & 7; };
+
+    unsigned avoff[JA];
+#pragma unroll
+    for (int j = 0; j < JA; ++j) {
+        const int L = WAVES * j + wave, plane = L / PARTS, part = L % PARTS;
+        const int q = (part * 64 + lane) >> 3, pl = lane & 7;
+        const unsigned s = min(s0 + (unsigned)(q >> 1), shi);
+        avoff[j] = (unsigned)((slot_voxel(s, plane, q & 1) - vb0) * (long)csz) +
+                   ((pl ^ swz_g(plane, q)) << 4);
+    }
+    unsigned bvoff[JW];
+#pragma unroll
+    for (int j = 0; j < JW; ++j) {
+        const int L = WAVES * j + wave, kx = L / BPARTS, part = L % BPARTS;
+        const int slot = part * 64 + lane, row = slot >> 3, pl = slot & 7;
+        bvoff[j] = (unsigned)((long)row * K * (long)sizeof(T)) + (unsigned)kx * csz +
+                   ((pl ^ swz<128>(row)) << 4);
+    }
+    const int r16 = lane & 15, g = lane >> 4;
+    int aoff[IB][3][2], boff[JB][2];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+        const long m = min(mb + wave * WTM + 16 * i + r16, p.M - 1);
+        const unsigned wi = (unsigned)(m >> 3);
+        const int w = (int)(m & 7), plane = w >> 1;
+        const int q0 = 2 * (int)(wi + row_div(wi, p) - s0) + (w & 1);
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                aoff[i][kx][h] = (plane * kPoolPlaneG + q0 + kx) * 128 +
+                                 (((g + 4 * h) ^ swz_g(plane, q0 + kx)) << 4);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int row = 16 * j + r16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) boff[j][h] = row * 128 + (((g + 4 * h) ^ swz<128>(row)) << 4);
+    }
+
+    // group walk (unit, kz, ky): input offset kz*dz + ky*dy + unit*128, weight
+    // offset (3 (3 kz + ky)) taps + unit*128
+    const int units = (int)(csz / 128u), ng = 9 * units;
+    const unsigned dy_b = (unsigned)D * csz, dz_b = dy_b * (unsigned)D;
+    unsigned a_off = 0, b_off = 0;
+    int a_kz = 0, a_ky = 0;
+    auto issue = [&](unsigned char *abuf, unsigned char *bbuf) {
+#pragma unroll
+        for (int j = 0; j < JA; ++j) {
+            const int L = WAVES * j + wave;
+            copy16_to_lds(a_base, a_off, avoff[j],
+                          abuf + (L / PARTS) * (kPoolPlaneG * 128) + (L % PARTS) * 1024);
+        }
+#pragma unroll
+        for (int j = 0; j < JW; ++j) {
+            const int L = WAVES * j + wave;
+            copy16_to_lds(b_base, b_off, bvoff[j],
+                          bbuf + (L / BPARTS) * (BN * 128) + (L % BPARTS) * 1024);
+        }
+        if (++a_ky < 3) {
+            a_off += dy_b;
+            b_off += 3 * csz;
+        } else {
+            a_ky = 0;
+            a_off -= 2 * dy_b;
+            if (++a_kz < 3) {
+                a_off += dz_b;
+                b_off += 3 * csz;
+            } else {
+                a_kz = 0;
+                a_off = a_off - 2 * dz_b + 128u;
+                b_off = b_off - 24 * csz + 128u;
+            }
+        }
+    };
+
+    floatx4 acc[IB][JB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto compute = [&](const unsigned char *abuf, const unsigned char *bbuf) {
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint4 a[IB], b[JB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i)
+                    a[i] = *reinterpret_cast<const uint4 *>(abuf + aoff[i][kx][h]);
+#pragma unroll
+                for (int j = 0; j < JB; ++j)
+                    b[j] = *reinterpret_cast<const uint4 *>(bbuf + kx * (BN * 128) + boff[j][h]);
+                Mfma<T>::chunk(acc, a, b);
+            }
+    };
+    // group gi in buffers gi & 1; the copies of group gi + 1 go to the other
+    // pair, last read by group gi - 1 (before this group's barrier)
+    auto group = [&](const int gi, unsigned char *acur, unsigned char *bcur, unsigned char *anxt,
+                     unsigned char *bnxt) {
+        wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        if (gi + 1 < ng) issue(anxt, bnxt);
+        compute(acur, bcur);
+    };
+    issue(A0, B0);
+    for (int gi = 0; gi < ng; gi += 2) {
+        group(gi, A0, B0, A1, B1);
+        if (gi + 1 < ng) group(gi + 1, A1, B1, A0, B0);
     }
     conv_epilogue<OT, IB, JB, true, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
 }
@@ -1533,6 +1695,17 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
             }
             TDM_CONV_DMA(4, 64, 64, 128, 2);
             break;
+        case 15:
+            // eight-wave pooled tap groups: the same conditions as 14
+            if constexpr (sizeof(T) == 2) {
+                if (splits == 1 && d.pool && p.small_rows) {
+                    dim3 grid((unsigned)((p.M + 8 * kPoolWinG - 1) / (8 * kPoolWinG)), d.N / 64, 1);
+                    conv_pool3g_kernel<T, OT><<<grid, 512, 0, s>>>(p);
+                    break;
+                }
+            }
+            TDM_CONV_DMA(4, 64, 64, 128, 2);
+            break;
         case 0: TDM_CONV_LDS(2, 64, 32, 128); break;
         case 1: TDM_CONV_LDS(2, 64, 64, 128); break;
         case 2: TDM_CONV_LDS(4, 64, 64, 128); break;
@@ -1610,12 +1783,14 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
 // Tile variants: {waves along m, wave tile m, wave tile n, stage bytes};
 // 0-8 stage through VGPRs (conv_lds_kernel), 9-13 copy straight into LDS
 // (conv_dma_kernel; 9-11 with three stage buffers, 12-13 with two), 14 the
-// pooled tap-group kernel (conv_pool3_kernel).
+// pooled tap-group kernel (conv_pool3_kernel), 15 its eight-wave form
+// (conv_pool3g_kernel).
 namespace {
 constexpr int kVariants[kConvVariants][4] = {
     {2, 64, 32, 128},  {2, 64, 64, 128}, {4, 64, 64, 128}, {2, 128, 32, 128}, {2, 128, 64, 128},
     {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64},  {4, 64, 64, 64},
-    {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}, {4, 48, 64, 128}};
+    {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}, {4, 48, 64, 128},
+    {4, 96, 64, 128}};  // 15: eight waves of 48 rows, listed as a 384 x 64 tile
 }
 
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
@@ -1626,7 +1801,8 @@ bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     // pooled tap groups: 16-bit, a fused pool after a 3x3x3 window, and
     // pooled extents of at least 12 (a tile's 24 windows then read at most
     // 27 slots of their window lines)
-    if (v == 14 && (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 || d.OD / 2 < 12))
+    if ((v == 14 || v == 15) &&
+        (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 || d.OD / 2 < 12))
         return false;
     if (v >= 9) {
         // conv_dma_kernel addresses a tile's input rows and weight rows with
