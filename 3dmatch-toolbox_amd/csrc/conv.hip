@@ -178,6 +178,27 @@ __device__ inline void store<fp8_t>(fp8_t *p, float v) {
     *p = to_fp8(v);
 }
 
+// Four consecutive values in one store (p aligned to 4 values).
+template <typename OT>
+__device__ inline void store4(OT *p, const float (&v)[4]) {
+    if constexpr (sizeof(OT) == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        OT o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) store<OT>(&o[r], v[r]);
+        if constexpr (sizeof(OT) == 2) {
+            uint2 u;
+            __builtin_memcpy(&u, o, 8);
+            *reinterpret_cast<uint2 *>(p) = u;
+        } else {
+            unsigned u;
+            __builtin_memcpy(&u, o, 4);
+            *reinterpret_cast<unsigned *>(p) = u;
+        }
+    }
+}
+
 // 16-bit storage bits of a float for the MFMA operand types.
 template <typename T>
 __device__ inline unsigned short bits16(float v);
@@ -1474,7 +1495,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0);
+                    acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[j], av[i], acc[j][i], 0, 0, 0);
         }
     } else {
         // One k-step of 32 taps: lane group g supplies taps 8g .. 8g+7.  The
@@ -1515,15 +1536,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) bf[j] = ld16(w + (long)(n0 + 16 * j + r16) * KP + 8 * g);
-        Mfma<T>::chunk(acc, af, bf);
+        Mfma<T>::chunk(acc, bf, af);  // acc[j][i]: the weights are the first operand
     }
 
-    // Epilogue through LDS: the accumulator layout scatters 16-column pieces
-    // of 4 rows per lane; staging the wave's 64x64 tile lets every lane write
-    // whole 16-byte pieces of contiguous output rows (conv1 is bound by its
-    // output writes: 28^3 x 64 values per patch).
-    // Float outputs go in two 32-column halves, so every output type stages
-    // 36 KB per workgroup (four workgroups per CU).
+    // Epilogue through LDS.  With the weights as the MFMAs' first operand,
+    // acc[j][i] gives the lane four consecutive channels (16j + 4g ..) of row
+    // 16i + r16, which go to LDS in one 4-value store; staging the wave's
+    // 64x64 tile then lets every lane write whole 16-byte pieces of contiguous
+    // output rows (conv1 is bound by its output writes: 28^3 x 64 values per
+    // patch).  Float outputs go in two 32-column halves, so every output type
+    // stages 36 KB per workgroup (four workgroups per CU).
     constexpr int HALVES = sizeof(OT) == 4 ? 2 : 1;
     constexpr int COLS = 64 / HALVES;
     constexpr int PITCH = COLS + 16 / (int)sizeof(OT);  // padded row, in OT elements
@@ -1536,17 +1558,19 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 4) void conv1_kernel(Conv
     for (int h = 0; h < HALVES; ++h) {
 #pragma unroll
         for (int j = h * (4 / HALVES); j < (h + 1) * (4 / HALVES); ++j) {
-            const int cl = 16 * j + r16;
-            const float bv = a.bias[n0 + cl];
+            const int c0 = 16 * j + 4 * g;
+            const float4 bv = *reinterpret_cast<const float4 *>(a.bias + n0 + c0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
+                float v[4] = {acc[j][i][0] + bv.x, acc[j][i][1] + bv.y, acc[j][i][2] + bv.z,
+                              acc[j][i][3] + bv.w};
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    float v = acc[i][j][r] + bv;
-                    if (a.relu) v = v > 0.0f ? v : 0.0f;
-                    if (sizeof(OT) == 1) v *= a.oscale;
-                    store<OT>(t + (16 * i + 4 * g + r) * PITCH + cl - h * COLS, v);
+                    if (a.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+                    if (sizeof(OT) == 1) v[r] *= a.oscale;
                 }
+                store4<OT>(t + (16 * i + r16) * PITCH + c0 - h * COLS, v);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -1876,6 +1900,9 @@ int launch_conv1(DType dt, bool out_f32, const VolumeSource &src, int N, bool re
                  const void *w, const float *bias, void *out, hipStream_t s, float oscale) {
     TDM_CHECK(N % 64 == 0, TDM_ERR_UNSUPPORTED, "first convolution needs a multiple of 64 outputs");
     TDM_CHECK(src.P >= 3, TDM_ERR_INVALID_ARG, "patch extent %d", src.P);
+    // the kernel reads four consecutive biases at a time
+    TDM_CHECK(reinterpret_cast<uintptr_t>(bias) % 16 == 0, TDM_ERR_INVALID_ARG,
+              "first convolution's bias is not 16-byte aligned");
     if (batch <= 0) return TDM_OK;
     Conv1Args a;
     a.base = src.base;
