@@ -1822,11 +1822,12 @@ bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     const int bn = (4 / kVariants[v][0]) * kVariants[v][2];
     const int stage_k = kVariants[v][3] / (int)dtype_size(dt);
     if (d.N % bn != 0 || d.C % stage_k != 0) return false;
-    // pooled tap groups: 16-bit, a fused pool after a 3x3x3 window, and
-    // pooled extents of at least 12 (a tile's 24 windows then read at most
-    // 27 slots of their window lines)
-    if ((v == 14 || v == 15) &&
-        (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 || d.OD / 2 < 12))
+    // pooled tap groups: 16-bit, a fused pool after a 3x3x3 window, pooled
+    // extents of at least 12 (a tile's windows then read at most 27 / 53
+    // slots of their window lines), and one 128-byte channel unit (the only
+    // form the descriptor net has, and so the only one tested)
+    if ((v == 14 || v == 15) && (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 ||
+                                 d.OD / 2 < 12 || d.C * (int)dtype_size(dt) != 128))
         return false;
     if (v >= 9) {
         // conv_dma_kernel addresses a tile's input rows and weight rows with
