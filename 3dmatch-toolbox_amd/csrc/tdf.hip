@@ -632,41 +632,70 @@ __global__ void copy_floats_kernel(const float *__restrict__ src, float *__restr
 // one that gives j its exact TDF (an occupied j is within reach at distance
 // 0), so v gets 1 if occupied, else exact[j]; decodes outside the grid are
 // searched here around the decoded voxel (core/demo.cu:44-56, bounds clipped).
+// Rows of voxels per thread in tdf_refgather_kernel.
+constexpr int kRefRows = 4;
+
 __global__ void tdf_refgather_kernel(const unsigned *__restrict__ bits, int dx, int dy, int dz,
                                      int r, Lut lut, const float *__restrict__ exact,
                                      float *__restrict__ tdf) {
-    // one workgroup per 256 x of a row: the true coordinates need only two
-    // (uniform) divisions per workgroup
+    // One workgroup per 256 x of kRefRows consecutive rows (z, y): a thread
+    // takes the same x in each, the row coordinates need only (uniform)
+    // divisions, and the loads of all kRefRows voxels go out before any is
+    // used -- with one voxel per thread a wave made a single round trip to a
+    // grid that does not fit in L2 and the kernel ran on latency.
     const long rw = row_words(dx);
     const unsigned nbx = (unsigned)(dx + 255) / 256;
-    const unsigned row = blockIdx.x / nbx;
-    const int tx = (int)(blockIdx.x - row * nbx) * 256 + (int)threadIdx.x;
-    const int ty = (int)(row % (unsigned)dy), tz = (int)(row / (unsigned)dy);
+    const unsigned rb = blockIdx.x / nbx;
+    const int tx = (int)(blockIdx.x - rb * nbx) * 256 + (int)threadIdx.x;
     if (tx >= dx) return;
-    const long v = ((long)tz * dy + ty) * dx + tx;
-    {
-        int x, y, z;
-        float_decode(v, dx, dy, x, y, z);
+    const long nrows = (long)dy * dz;
+    long v[kRefRows];
+    int x[kRefRows], y[kRefRows], z[kRefRows], ty[kRefRows], tz[kRefRows];
+    bool in[kRefRows];
+    unsigned word[kRefRows];
+    float ej[kRefRows];
+#pragma unroll
+    for (int k = 0; k < kRefRows; ++k) {
+        const long row = min((long)rb * kRefRows + k, nrows - 1);
+        ty[k] = (int)(row % dy);
+        tz[k] = (int)(row / dy);
+        v[k] = row * dx + tx;
+        float_decode(v[k], dx, dy, x[k], y[k], z[k]);
+        // the word holding v's occupancy bit, and the exact TDF at the
+        // decoded voxel j (at v when the decode leaves the grid)
+        in[k] = x[k] >= 0 && x[k] < dx && y[k] >= 0 && y[k] < dy && z[k] >= 0 && z[k] < dz;
+        word[k] = bits[row * rw + (tx >> 5)];
+        ej[k] = exact[in[k] ? ((long)z[k] * dy + y[k]) * dx + x[k] : v[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kRefRows; ++k) {
+        if ((long)rb * kRefRows + k >= nrows) break;
         float val;
-        if (x == tx && y == ty && z == tz) {
-            val = exact[v];
-        } else if (bit_at(bits, rw, dy, tx, ty, tz)) {
+        if (x[k] == tx && y[k] == ty[k] && z[k] == tz[k]) {
+            val = ej[k];
+        } else if ((word[k] >> (tx & 31)) & 1u) {
             val = 1.0f;
-        } else if (x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz) {
-            val = exact[((long)z * dy + y) * dx + x];
+        } else if (in[k]) {
+            val = ej[k];
         } else {
             int best = kLutSize;
-            for (int ix = max(0, x - r); ix < min(dx, x + r + 1); ++ix)
-                for (int iy = max(0, y - r); iy < min(dy, y + r + 1); ++iy)
-                    for (int iz = max(0, z - r); iz < min(dz, z + r + 1); ++iz)
+            for (int ix = max(0, x[k] - r); ix < min(dx, x[k] + r + 1); ++ix)
+                for (int iy = max(0, y[k] - r); iy < min(dy, y[k] + r + 1); ++iy)
+                    for (int iz = max(0, z[k] - r); iz < min(dz, z[k] + r + 1); ++iz)
                         if (bit_at(bits, rw, dy, ix, iy, iz)) {
-                            int d2 = (x - ix) * (x - ix) + (y - iy) * (y - iy) + (z - iz) * (z - iz);
+                            const int d2 = (x[k] - ix) * (x[k] - ix) + (y[k] - iy) * (y[k] - iy) +
+                                           (z[k] - iz) * (z[k] - iz);
                             best = min(best, d2);
                         }
             val = best < kLutSize ? lut.v[best] : 0.0f;
         }
-        tdf[v] = val;
+        tdf[v[k]] = val;
     }
+}
+
+// Workgroups of tdf_refgather_kernel for a dx x dy x dz grid.
+inline unsigned refgather_blocks(int dx, int dy, int dz) {
+    return (unsigned)((long)((dx + 255) / 256) * (((long)dy * dz + kRefRows - 1) / kRefRows));
 }
 
 // ---- patch gather --------------------------------------------------------------------
@@ -1141,9 +1170,9 @@ int tdf_from_bits(const unsigned *bits, int dx, int dy, int dz, int r, int decod
     }
     TDM_LAUNCHED();
     if (decode == TDM_DECODE_REFERENCE) {
-        // at most (dx/256 + 1) * dy * dz <= 2^31 workgroups
-        const unsigned blocks = (unsigned)((long)((dx + 255) / 256) * dy * dz);
-        tdf_refgather_kernel<<<blocks, 256, 0, s>>>(bits, dx, dy, dz, r, lut, work, tdf);
+        // at most (dx/256 + 1) * dy * dz / kRefRows < 2^31 workgroups
+        tdf_refgather_kernel<<<refgather_blocks(dx, dy, dz), 256, 0, s>>>(bits, dx, dy, dz, r,
+                                                                          lut, work, tdf);
         TDM_LAUNCHED();
     }
     return TDM_OK;
@@ -1660,9 +1689,8 @@ int tdm_tdf_plan_run(tdm_tdf_plan *plan, void *stream) {
             const unsigned cb = (unsigned)std::min<long>((n / 4 + 255) / 256, 16384);
             copy_floats_kernel<<<cb, 256, 0, s>>>(d.tdf, plan->ref_exact, n);
             TDM_LAUNCHED();
-            const unsigned blocks = (unsigned)((long)((dx + 255) / 256) * dy * dz);
-            tdf_refgather_kernel<<<blocks, 256, 0, s>>>(plan->bits + d.word_off, dx, dy, dz,
-                                                         plan->r, lut, plan->ref_exact, d.tdf);
+            tdf_refgather_kernel<<<refgather_blocks(dx, dy, dz), 256, 0, s>>>(
+                plan->bits + d.word_off, dx, dy, dz, plan->r, lut, plan->ref_exact, d.tdf);
             TDM_LAUNCHED();
         }
     return TDM_OK;
