@@ -115,17 +115,22 @@ def test_descriptors_vs_fp64(torch_gpu, net_files, patches, fp64, prec, tol):
 
 
 def test_grid_path_equals_patch_path(torch_gpu, net_files, patches):
-    """Patch gather fused into conv1 gives bit-identical descriptors."""
+    """Patch gather fused into conv1 gives bit-identical descriptors, also for
+    keypoints whose patch reaches beyond the grid (zeros outside, the gather's
+    bounds-checked path)."""
     js, wt, _ = net_files
-    p, tdf, dims, kg = patches
+    p0, tdf, dims, kg0 = patches
+    border = np.array([[0, 0, 0], [dims[0] - 1, dims[1] - 1, dims[2] - 1], [14, 15, 16]], np.int32)
+    kg = np.ascontiguousarray(np.concatenate([kg0, border]))
+    p = oracle.extract_patches(tdf, kg)
     for prec in (tdm.PREC_FP32, tdm.PREC_BF16, tdm.PREC_FP16, tdm.PREC_FP8):
-        net = _make_net(torch_gpu, js, wt, prec, p)
+        net = _make_net(torch_gpu, js, wt, prec, p0)
         a = torch_gpu.empty(len(p), 512, dtype=torch_gpu.float32, device="cuda:0")
         b = torch_gpu.empty_like(a)
         net.forward(torch_gpu.from_numpy(p).to("cuda:0"), a)
         net.forward_grid(torch_gpu.from_numpy(tdf).to("cuda:0"), dims,
                          torch_gpu.from_numpy(kg).to("cuda:0"), b)
-        assert torch_gpu.equal(a, b)
+        assert torch_gpu.equal(a, b), prec
 
 
 def test_batch_invariance(torch_gpu, net_files, patches):
@@ -169,6 +174,8 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
         tdf = oracle.compute_tdf(occ, vs, np.float32(0.05))
         idx = synth.keypoint_indices(len(cloud), 5 + 2 * k, seed=50 + k)
         kg = oracle.keypoint_grid(cloud[idx], origin, vs)
+        if k == 1:  # a patch reaching beyond the grid's corner
+            kg = np.ascontiguousarray(np.concatenate([kg, np.array([[1, 2, 0]], np.int32)]))
         grids.append((torch.from_numpy(tdf).to("cuda:0"), dims))
         kps.append(kg)
         host_tdfs.append(tdf)
