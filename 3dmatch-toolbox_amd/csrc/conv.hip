@@ -1143,6 +1143,183 @@ __global__ __launch_bounds__(256, 2) void conv_pool3_kernel(ConvArgs p) {
     conv_epilogue<OT, IB, JB, true, false>(acc, p, mb + wave * WTM, nb, lane, 0u);
 }
 
+// conv_pool3_kernel with resident workgroups (two per CU) that walk a list of
+// tiles, for layers of one 128-byte channel unit (9 tap groups, 27 steps): a
+// tile's last group issues the next tile's first input rows and weights into
+// the buffers it has just freed, and the next tile's copy offsets are worked
+// out while this tile's MFMAs run, so a tile no longer starts by waiting for
+// its first copies.  XCD x (workgroups w with w % 8 == x) walks one contiguous
+// run of the tile sequence, as xcd_tile arranges for the one-shot kernels.
+// Tiles, staging, reduction order and epilogue are conv_pool3_kernel's
+// (bit-identical).
+template <typename T, typename OT>
+__global__ __launch_bounds__(256, 2) void conv_pool3p_kernel(ConvArgs p, int ntx, int ntiles) {
+    static_assert(sizeof(T) == 2, "16-bit operands");
+    constexpr int BM = 8 * kPoolWin, BN = 64, WTM = BM / 4, IB = WTM / 16, JB = BN / 16;
+    constexpr int ASZ = 4 * kPoolPlane * 128, BSZ = BN * 128;
+    constexpr int JA = 4 * kPoolStaged * 8 / 256, JW = BN * 8 / 256, PARTS = kPoolStaged * 8 / 64;
+    constexpr int NG = 9, NSTEPS = 27;  // one channel unit
+    __shared__ __attribute__((aligned(16))) unsigned char A0[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char A1[ASZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B0[BSZ];
+    __shared__ __attribute__((aligned(16))) unsigned char B1[BSZ];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int C = p.C, D = p.D, PD = p.OD >> 1;
+    const unsigned csz = (unsigned)C * (unsigned)sizeof(T);
+    const long K = (long)p.taps * C;
+    const unsigned dy_b = (unsigned)D * csz, dz_b = dy_b * (unsigned)D;
+    const int r16 = lane & 15, g = lane >> 4;
+
+    // this workgroup's tiles: first + k * stride within its XCD's run
+    const int xcd = (int)(blockIdx.x % kXcds), per_xcd = (int)(gridDim.x / kXcds);
+    const int run = (ntiles + (int)kXcds - 1) / (int)kXcds;
+    const int run_end = min((xcd + 1) * run, ntiles);
+    const int first = xcd * run + (int)(blockIdx.x / kXcds);
+    if (first >= run_end) return;  // whole workgroup
+
+    struct Tl {
+        long mb;
+        int nb;
+        const unsigned char *a_base, *b_base;
+        unsigned avoff[JA];
+        int aoff[IB][3][2];
+    };
+    auto setup = [&](int t, Tl &s) {
+        const int tx = t % ntx, ty = t / ntx;
+        s.mb = (long)tx * BM;
+        s.nb = ty * BN;
+        const unsigned nwin = (unsigned)(p.M >> 3), wi0 = (unsigned)(s.mb >> 3);
+        const unsigned wl = min(wi0 + (unsigned)kPoolWin, nwin) - 1;
+        const unsigned s0 = wi0 + row_div(wi0, p), shi = wl + row_div(wl, p) + 1;
+        auto slot_voxel = [&](unsigned sl, int plane, int xo) -> long {
+            const unsigned line = slot_div(sl, p);
+            const int px = (int)(sl - line * (unsigned)(PD + 1));
+            const unsigned q1 = row_div(line, p), q2 = row_div(q1, p);
+            const int py = (int)(line - q1 * (unsigned)PD), pz = (int)(q1 - q2 * (unsigned)PD);
+            return (((long)q2 * D + 2 * pz + (plane >> 1)) * D + 2 * py + (plane & 1)) * (long)D +
+                   2 * px + xo;
+        };
+        const long vb0 = slot_voxel(s0, 0, 0);
+        s.a_base = static_cast<const unsigned char *>(p.in) + vb0 * (long)csz;
+        s.b_base = static_cast<const unsigned char *>(p.w) + (long)s.nb * K * (long)sizeof(T);
+#pragma unroll
+        for (int j = 0; j < JA; ++j) {
+            const int L = 4 * j + wave, plane = L / PARTS, part = L % PARTS;
+            const int q = (part * 64 + lane) >> 3, pl = lane & 7, r = plane * kPoolPlane + q;
+            const unsigned sl = min(s0 + (unsigned)(q >> 1), shi);
+            s.avoff[j] = (unsigned)((slot_voxel(sl, plane, q & 1) - vb0) * (long)csz) +
+                         ((pl ^ swz_plane(r)) << 4);
+        }
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            const long m = min(s.mb + wave * WTM + 16 * i + r16, p.M - 1);
+            const unsigned wi = (unsigned)(m >> 3);
+            const int w = (int)(m & 7);
+            const int r0 = (w >> 1) * kPoolPlane + 2 * (int)(wi + row_div(wi, p) - s0) + (w & 1);
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    s.aoff[i][kx][h] = (r0 + kx) * 128 + (((g + 4 * h) ^ swz_plane(r0 + kx)) << 4);
+        }
+    };
+    unsigned bvoff[JW];
+#pragma unroll
+    for (int j = 0; j < JW; ++j) {
+        const int slot = (4 * j + wave) * 64 + lane, row = slot >> 3, pl = slot & 7;
+        bvoff[j] = (unsigned)((long)row * K * (long)sizeof(T)) + ((pl ^ swz<128>(row)) << 4);
+    }
+    int boff[JB][2];
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+        const int row = 16 * j + r16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) boff[j][h] = row * 128 + (((g + 4 * h) ^ swz<128>(row)) << 4);
+    }
+    // group g's input rows (kz = g / 3, ky = g % 3) and step t's weights (tap t)
+    auto issue_a = [&](unsigned char *buf, const Tl &s, int gr) {
+        const unsigned off = (unsigned)(gr / 3) * dz_b + (unsigned)(gr % 3) * dy_b;
+#pragma unroll
+        for (int j = 0; j < JA; ++j) {
+            const int L = 4 * j + wave;
+            copy16_to_lds(s.a_base, off, s.avoff[j],
+                          buf + (L / PARTS) * (kPoolPlane * 128) + (L % PARTS) * 1024);
+        }
+    };
+    auto issue_b = [&](unsigned char *buf, const Tl &s, int st) {
+#pragma unroll
+        for (int j = 0; j < JW; ++j)
+            copy16_to_lds(s.b_base, (unsigned)st * csz, bvoff[j], buf + (4 * j + wave) * 1024);
+    };
+
+    Tl cur, nxt;
+    floatx4 acc[IB][JB];
+    auto compute = [&](const unsigned char *abuf, const unsigned char *bbuf, const int kx) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint4 a[IB], b[JB];
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+                a[i] = *reinterpret_cast<const uint4 *>(abuf + cur.aoff[i][kx][h]);
+#pragma unroll
+            for (int j = 0; j < JB; ++j) b[j] = *reinterpret_cast<const uint4 *>(bbuf + boff[j][h]);
+            Mfma<T>::chunk(acc, a, b);
+        }
+    };
+    // one tile; its groups alternate between (aev, bev) and (aod, bod) from
+    // group 0 (steps alike), and the next tile's group 0 goes to the pair its
+    // group 8 does not use
+    auto tile = [&](unsigned char *aev, unsigned char *aod, unsigned char *bev, unsigned char *bod,
+                    const bool has_next, const int tn) {
+#pragma unroll
+        for (int gi = 0; gi < NG; ++gi) {
+            unsigned char *acur = (gi & 1) ? aod : aev, *anxt = (gi & 1) ? aev : aod;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int t = 3 * gi + kx;
+                unsigned char *bcur = (t & 1) ? bod : bev, *bnxt = (t & 1) ? bev : bod;
+                // at kx = 1 the group copy issued at kx = 0 (after this
+                // step's weights) may still be in flight
+                if (kx == 1 && (gi + 1 < NG || has_next)) wait_vmcnt<JA>();
+                else wait_vmcnt<0>();
+                __builtin_amdgcn_s_barrier();
+                if (t + 1 < NSTEPS) issue_b(bnxt, cur, t + 1);
+                else if (has_next) issue_b(bnxt, nxt, 0);
+                if (kx == 0) {
+                    if (gi + 1 < NG) issue_a(anxt, cur, gi + 1);
+                    else if (has_next) issue_a(anxt, nxt, 0);
+                }
+                compute(acur, bcur, kx);
+                // the next tile's offsets, behind the first step's MFMAs
+                if (t == 0 && has_next) setup(tn, nxt);
+            }
+        }
+    };
+    setup(first, cur);
+    issue_b(B0, cur, 0);
+    issue_a(A0, cur, 0);
+    for (int t = first; t < run_end;) {
+        // two tiles per iteration: the buffer pairs swap roles each tile
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+            const int tn = t + per_xcd;
+            const bool has_next = tn < run_end;
+#pragma unroll
+            for (int i = 0; i < IB; ++i)
+#pragma unroll
+                for (int j = 0; j < JB; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+            if (par == 0) tile(A0, A1, B0, B1, has_next, tn);
+            else tile(A1, A0, B1, B0, has_next, tn);
+            conv_epilogue<OT, IB, JB, true, false>(acc, p, cur.mb + wave * WTM, cur.nb, lane, 0u);
+            t = tn;
+            if (!has_next) break;
+            cur = nxt;
+        }
+    }
+}
+
 // The pooled tap-group kernel with one workgroup of eight waves per CU and a
 // whole (kz, ky) group -- its input rows and the weights of its three kx taps
 // -- per stage buffer: 48 windows x 64 columns per tile, eight waves of 48
@@ -1650,6 +1827,21 @@ __global__ void maxpool_kernel(const T *__restrict__ in, long batch, int C, int 
     }
 }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 template <typename T, typename OT>
 int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void *w,
                   const float *bias, void *out, int splits, float *partial, int variant,
@@ -1714,6 +1906,20 @@ int conv_dispatch(const tdm::ConvDesc &d, int batch, const void *in, const void 
                 if (splits == 1 && d.pool && p.small_rows) {
                     dim3 grid((unsigned)((p.M + 8 * kPoolWin - 1) / (8 * kPoolWin)), d.N / 64, 1);
                     conv_pool3_kernel<T, OT><<<grid, 256, 0, s>>>(p);
+                    break;
+                }
+            }
+            TDM_CONV_DMA(4, 64, 64, 128, 2);
+            break;
+        case 16:
+            // resident pooled tap groups: the same conditions as 14, two
+            // workgroups per CU (a multiple of the 8 XCDs)
+            if constexpr (sizeof(T) == 2) {
+                if (splits == 1 && d.pool && p.small_rows) {
+                    const int ntx = (int)((p.M + 8 * kPoolWin - 1) / (8 * kPoolWin));
+                    const int ntiles = ntx * (d.N / 64);
+                    const unsigned wgs = (unsigned)(2 * device_cus()) / kXcds * kXcds;
+                    conv_pool3p_kernel<T, OT><<<wgs, 256, 0, s>>>(p, ntx, ntiles);
                     break;
                 }
             }
@@ -1808,13 +2014,14 @@ int conv_reduction_steps(DType dt, const ConvDesc &d) {
 // 0-8 stage through VGPRs (conv_lds_kernel), 9-13 copy straight into LDS
 // (conv_dma_kernel; 9-11 with three stage buffers, 12-13 with two), 14 the
 // pooled tap-group kernel (conv_pool3_kernel), 15 its eight-wave form
-// (conv_pool3g_kernel).
+// (conv_pool3g_kernel), 16 its resident-workgroup form (conv_pool3p_kernel).
 namespace {
 constexpr int kVariants[kConvVariants][4] = {
     {2, 64, 32, 128},  {2, 64, 64, 128}, {4, 64, 64, 128}, {2, 128, 32, 128}, {2, 128, 64, 128},
     {1, 64, 64, 128},  {4, 64, 64, 64},  {2, 64, 64, 64},  {2, 128, 32, 64},  {4, 64, 64, 64},
     {2, 64, 64, 64},   {2, 128, 32, 64}, {2, 64, 64, 128}, {4, 64, 64, 128}, {4, 48, 64, 128},
-    {4, 96, 64, 128}};  // 15: eight waves of 48 rows, listed as a 384 x 64 tile
+    {4, 96, 64, 128},   // 15: eight waves of 48 rows, listed as a 384 x 64 tile
+    {4, 48, 64, 128}};  // 16: 14's tiles, resident workgroups
 }
 
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
@@ -1826,7 +2033,7 @@ bool conv_variant_valid(DType dt, int v, const ConvDesc &d) {
     // extents of at least 12 (a tile's windows then read at most 27 / 53
     // slots of their window lines), and one 128-byte channel unit (the only
     // form the descriptor net has, and so the only one tested)
-    if ((v == 14 || v == 15) && (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 ||
+    if ((v == 14 || v == 15 || v == 16) && (dtype_size(dt) != 2 || !d.pool || d.ksize != 3 ||
                                  d.OD / 2 < 12 || d.C * (int)dtype_size(dt) != 128))
         return false;
     if (v >= 9) {

@@ -47,9 +47,9 @@ int launch_conv(DType dt, bool out_f32, const ConvDesc &d, int batch, const void
 
 // Tile variants of the LDS kernel (0 .. kConvVariants-1): whether one fits
 // a layer (its N is a multiple of the tile's BN), the default, and the tile.
-// Variants 14 and 15 are the pooled tap-group kernels (16-bit pooled 3x3x3
+// Variants 14-16 are the pooled tap-group kernels (16-bit pooled 3x3x3
 // layers).
-constexpr int kConvVariants = 16;
+constexpr int kConvVariants = 17;
 bool conv_variant_valid(DType dt, int v, const ConvDesc &d);
 int conv_default_variant(DType dt, const ConvDesc &d);
 void conv_variant_tile(int v, int *bm, int *bn);

@@ -198,7 +198,7 @@ def test_multi_grid_pass_equals_per_grid(torch_gpu, net_files):
         assert torch.equal(got, want), prec
 
 
-NUM_VARIANTS = 16  # conv tile variants (tdm_net_set_layer_variant)
+NUM_VARIANTS = 17  # conv tile variants (tdm_net_set_layer_variant)
 
 
 def test_tile_variants_bit_identical_and_autotune(torch_gpu, net_files, patches):
