@@ -91,17 +91,21 @@ class DescriptorPipeline:
         self.net.forward_grid(self.tdf, dims, self.kp_dev, self.feat)
 
 
-def measured_traffic(precision):
-    """conv2's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (None if there is none for the
-    precision)."""
+def measured_traffic(precision, patches):
+    """conv2's HBM bytes per launch of `patches` patches, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (measured on a
+    launch of its stated size and scaled by patch count; None if there is no
+    summary for the precision)."""
     import re
 
     path = os.path.join(REPO, "profiles", f"hbm_traffic_descriptor_pipeline_{precision}.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"conv2 \(`[^`]*`\): (\d+) MB read and (\d+) MB written", open(path).read())
-    return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
+    m = re.search(r"conv2 \(`[^`]*`\): (\d+) MB read and (\d+) MB written per (\d+)-patch launch",
+                  open(path).read())
+    if not m:
+        return None
+    return (float(m.group(1)) + float(m.group(2))) * 1e6 * patches / float(m.group(3))
 
 
 def layer_tflops(net):
@@ -987,8 +991,8 @@ def main() -> int:
             "peak": MFMA_PEAK_TFLOPS[args.precision],
             "unit": "TFLOP/s",
             "frac": achieved / MFMA_PEAK_TFLOPS[args.precision],
-            "traffic": measured_traffic(args.precision),
-            "traffic_source": "profiles/hbm_traffic_descriptor_pipeline_<precision>.md (1000-patch launch)",
+            "traffic": measured_traffic(args.precision, patches2 / max(passes2, 1)),
+            "traffic_source": "profiles/hbm_traffic_descriptor_pipeline_<precision>.md, scaled to the average patches per launch",
             "flop_per_launch": conv2_flops * (patches2 / max(passes2, 1)),
             "avg_launch_ms": ms2 / max(passes2, 1),
         },
